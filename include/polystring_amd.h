@@ -1,0 +1,138 @@
+/*
+ * polystring_amd.h -- C ABI of the MI355X (gfx950) SCFT hot path.
+ *
+ * This is the drop-in boundary beneath PolyOrder's Updater / Propagator /
+ * Density / FieldAX / Model_AB interfaces: plain C, opaque handles, host
+ * pointers borrowed for the duration of a call, int status codes (0 = ok)
+ * and a thread-local message from po_last_error().  No exception crosses
+ * it.  All host arrays are FP64, C order (x, y, z), z fastest -- the blitz
+ * layout of the reference's Grid and Propagator::_qs (src/Propagator.cc:16).
+ *
+ * Reference interfaces each entry point replaces are cited in its comment
+ * (paths relative to the reference repository root).
+ */
+#ifndef POLYSTRING_AMD_H
+#define POLYSTRING_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PO_OK 0
+#define PO_ERR_INVALID 1      /* bad argument or unsupported shape */
+#define PO_ERR_DEVICE 2       /* HIP runtime error */
+#define PO_ERR_QUADRATURE 3   /* slice count invalid for the quadrature */
+#define PO_ERR_STATE 4        /* call out of order (e.g. no solve yet) */
+
+/* precision of the bulk propagator arithmetic */
+#define PO_FP64 0             /* everything FP64 */
+#define PO_FP32_BULK 1        /* FFTs, Boltzmann factors, slices in FP32; Q, H, Anderson in FP64 */
+
+/* MDE algorithms (AlgorithmMDEType, include/common.h) */
+#define PO_RQM4 0             /* src/RQM4.cc */
+#define PO_OS2 1              /* src/PseudoSpectral.cc */
+
+/* contour quadratures */
+#define PO_QUAD_SIMPSON 0     /* src/Simpson.cc */
+#define PO_QUAD_QUAD4C 1      /* src/Quadrature4.cc (Quad4_Closed) */
+#define PO_QUAD_TRAP 2        /* src/Density.cc:107-121 (Density::_update) */
+#define PO_QUAD_AUTO 3        /* Simpson for an odd slice count, else Quad4_Closed (src/Model_AB.cc:719-728) */
+
+/* fields of po_model_get */
+#define PO_FIELD_WA 0
+#define PO_FIELD_WB 1
+#define PO_FIELD_PHIA 2
+#define PO_FIELD_PHIB 3
+
+typedef struct po_ctx po_ctx;
+typedef struct po_prop po_prop;
+typedef struct po_model po_model;
+
+/* Grid + UnitCell description (Config [Grid]/[UnitCell] keys; UnitCell(const
+ * Config&), include/UnitCell.h:44-82).  crystal follows CrystalSystemType
+ * order (include/common.h:149-163): 0 LAMELLAR, 1 SQUARE, 2 RECTANGULAR,
+ * 3 HEXAGONAL, 4 OBLIQUE, 5 CUBIC, 6 TETRAGONAL, 7 ORTHORHOMBIC, 8 TRIGONAL,
+ * 9 MONOCLINIC.  Angles in radians. */
+typedef struct {
+    int dim, Lx, Ly, Lz;
+    int crystal;
+    double a, b, c, alpha, beta, gamma;
+} po_grid_desc;
+
+/* Model_AB parameters (src/Model_AB.cc:85-134).  Ms = [sA, sB] slice counts;
+ * fA and ds follow from them (src/Model_AB.cc:100-111).  lamA, lamB are the
+ * Anderson damping bases of w <- W + (1 - lam^{k+1}) D (src/FieldAX.cc:287). */
+typedef struct {
+    po_grid_desc grid;
+    int sA, sB;
+    double chiN;
+    double lamA, lamB;
+    int n_anderson;
+    int algo;      /* PO_RQM4 or PO_OS2 */
+    int contour;   /* PO_QUAD_AUTO (Simpson rule) or PO_QUAD_TRAP */
+} po_model_desc;
+
+/* ---------------------------------------------------------------- context */
+int po_version(void);
+const char *po_last_error(void);
+/* One GPU, one HIP stream, one precision.  No reference counterpart: the
+ * reference is single-threaded CPU code. */
+int po_ctx_create(int device, int precision, po_ctx **out);
+int po_ctx_destroy(po_ctx *ctx);
+int po_ctx_synchronize(po_ctx *ctx);
+/* 1 when (Lx, Ly, Lz) has compiled kernels */
+int po_shape_supported(int Lx, int Ly, int Lz);
+
+/* --------------------------------------------------- Propagator + Updater */
+/* Propagator(name, cfg, len, ds, up) with up = new RQM4(uc,Lx,Ly,Lz,ds) or
+ * new PseudoSpectral(uc,Lx,Ly,Lz,ds) (src/Propagator.cc:12-38,
+ * src/RQM4.cc:9-16, src/PseudoSpectral.cc:33-38).  Slices live on the GPU. */
+int po_prop_create(po_ctx *ctx, const po_grid_desc *grid, int algo, int len, double ds, po_prop **out);
+int po_prop_destroy(po_prop *p);
+int po_prop_len(const po_prop *p);
+/* Propagator::set_head(const Grid&) (src/Propagator.cc:62-67) */
+int po_prop_set_head(po_prop *p, const double *host_q0);
+/* qB->set_head(qA->get_tail()) without a host round trip (src/Model_AB.cc:140,144) */
+int po_prop_set_head_from_tail(po_prop *dst, const po_prop *src);
+/* Propagator::update(const Grid& w) -> Updater::solve(q, w): RQM4::solve
+ * (src/RQM4.cc:41-93) or PseudoSpectral::solve (src/PseudoSpectral.cc:69-104). */
+int po_prop_solve(po_prop *p, const double *host_w);
+/* Propagator::operator[](s) / get_tail() (src/Propagator.cc:48-60) */
+int po_prop_get_slice(const po_prop *p, int s, double *host_out);
+/* Propagator::Q(s) = mean of slice s (src/Propagator.cc:85-93) */
+int po_prop_Q(const po_prop *p, int s, double *Q);
+
+/* ---------------------------------------------------------------- Density */
+/* Density::update(q, qc, cc) -> Simpson::solve / Quad4_Closed::solve /
+ * Density::_update (src/Density.cc:83-121, src/Simpson.cc:14-45,
+ * src/Quadrature4.cc:17-53).  Returns PO_ERR_QUADRATURE where the reference
+ * prints an error and leaves the density untouched. */
+int po_density(const po_prop *q, const po_prop *qc, int quad, double cc, double *host_phi);
+
+/* ------------------------------------------------------ Model_AB + scft -- */
+/* Model_AB(config) with Anderson mixing, incompressible melt, File init of
+ * wA, wB (src/Model_AB.cc:85-134,633-644,713-783). */
+int po_model_create(po_ctx *ctx, const po_model_desc *desc, const double *wA0, const double *wB0, po_model **out);
+int po_model_destroy(po_model *m);
+/* n_iter calls of Model_AB::update() (src/Model_AB.cc:136-261), queued on the
+ * GPU with no host synchronisation. */
+int po_model_update(po_model *m, int n_iter);
+/* Observables as Model_AB::H(), Hw(), Hs(), residual_error() and qB->Qt()
+ * return them after the last update (src/Model_AB.cc:263-315); synchronises.
+ * out[0..4] = H, Hw, Hs, residual, Q. */
+int po_model_observe(po_model *m, double *out5);
+/* Current wA, wB (after the last update) or phiA, phiB (of the last update). */
+int po_model_get(po_model *m, int which, double *host_out);
+/* scft::relax with the save_data stopping rules (src/scft.cc:382-422,
+ * 532-587): up to max_iter updates; H and the residual are read every
+ * record_interval; stops on residual < thresh after min_iter (and at least
+ * two records) or on non-finite H.  *iters gets the iterations run. */
+int po_model_relax(po_model *m, int max_iter, int min_iter, int record_interval, double thresh, int *iters,
+                   double *H, double *residual);
+/* number of contour steps one update() advances (2 * Ns: qA, qB, qBc, qAc) */
+long po_model_contour_steps(const po_model *m);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,216 @@
+"""ctypes binding of the CPU FP64 oracle (oracle/po_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker.  The product path in
+polystring_amd/ never imports this module.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+# CrystalSystemType order, include/common.h:149-163
+CRYSTAL = {
+    "LAMELLAR": 0, "SQUARE": 1, "RECTANGULAR": 2, "HEXAGONAL": 3, "OBLIQUE": 4,
+    "CUBIC": 5, "TETRAGONAL": 6, "ORTHORHOMBIC": 7, "TRIGONAL": 8,
+    "MONOCLINIC": 9, "TRICLINIC": 10,
+}
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def _load():
+    if not os.path.exists(_LIB_PATH):
+        build()
+    lib = ctypes.CDLL(_LIB_PATH)
+    lib.or_fft1d.argtypes = [ctypes.c_int, _dp, ctypes.c_int]
+    lib.or_fft3d.argtypes = [ctypes.c_int] * 3 + [_dp, ctypes.c_int]
+    lib.or_cell_shape.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_double] * 6 + [_dp, _dp]
+    lib.or_cell_shape.restype = ctypes.c_int
+    lib.or_k2_orthogonal.argtypes = [ctypes.c_int, _dp] + [ctypes.c_int] * 3 + [_dp]
+    lib.or_klaplacian.argtypes = [_dp] + [ctypes.c_int] * 3 + [ctypes.c_double, _dp]
+    for f in (lib.or_rqm4_solve, lib.or_os2_solve):
+        f.argtypes = [ctypes.c_int] * 3 + [_dp, ctypes.c_double, ctypes.c_int, _dp, _dp]
+    for f in (lib.or_simpson, lib.or_quad4_closed, lib.or_trapezoid):
+        f.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_double, _dp]
+        f.restype = ctypes.c_int
+    lib.or_mean.argtypes = [ctypes.c_int, _dp]
+    lib.or_mean.restype = ctypes.c_double
+    lib.or_lu_solve.argtypes = [ctypes.c_int, _dp, _dp, _dp]
+    lib.or_lu_solve.restype = ctypes.c_int
+    lib.or_model_create.argtypes = ([ctypes.c_int] * 4 + [ctypes.c_int] + [ctypes.c_double] * 6
+                                    + [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                       _dp, _dp])
+    lib.or_model_create.restype = ctypes.c_void_p
+    lib.or_model_destroy.argtypes = [ctypes.c_void_p]
+    lib.or_model_update.argtypes = [ctypes.c_void_p]
+    for name in ("or_model_H", "or_model_Hw", "or_model_Hs", "or_model_residual", "or_model_Q"):
+        getattr(lib, name).argtypes = [ctypes.c_void_p]
+        getattr(lib, name).restype = ctypes.c_double
+    lib.or_model_get.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+    lib.or_model_relax.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_double, _dp, _dp]
+    lib.or_model_relax.restype = ctypes.c_int
+    lib.or_num_threads.restype = ctypes.c_int
+    lib.or_set_num_threads.argtypes = [ctypes.c_int]
+    return lib
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = _load()
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def fft1d(x, sign=-1):
+    a = np.ascontiguousarray(x, dtype=np.complex128).copy()
+    lib().or_fft1d(a.size, _p(a.view(np.float64)), sign)
+    return a
+
+
+def fft3d(x, sign=-1):
+    a = np.ascontiguousarray(x, dtype=np.complex128).copy()
+    Lx, Ly, Lz = a.shape
+    lib().or_fft3d(Lx, Ly, Lz, _p(a.view(np.float64)), sign)
+    return a
+
+
+def cell_shape(dim, cs, a, b=0.0, c=0.0, alpha=0.0, beta=0.0, gamma=0.0):
+    l = np.zeros(3)
+    g = np.zeros(9)
+    code = CRYSTAL[cs] if isinstance(cs, str) else int(cs)
+    rc = lib().or_cell_shape(dim, code, a, b, c, alpha, beta, gamma, _p(l), _p(g))
+    if rc != 0:
+        raise ValueError("bad cell parameters")
+    return l, g.reshape(3, 3)
+
+
+def k2_orthogonal(dim, l, shape):
+    out = np.zeros(shape)
+    l = np.ascontiguousarray(l, dtype=np.float64)
+    lib().or_k2_orthogonal(dim, _p(l), *shape, _p(out))
+    return out
+
+
+def klaplacian(g, shape, ds):
+    out = np.zeros(shape)
+    g = np.ascontiguousarray(g, dtype=np.float64).ravel()
+    lib().or_klaplacian(_p(g), *shape, ds, _p(out))
+    return out
+
+
+def rqm4_solve(k2neg, ds, w, head, length):
+    shape = w.shape
+    qs = np.zeros((length,) + shape)
+    qs[0] = head
+    lib().or_rqm4_solve(*shape, _p(np.ascontiguousarray(k2neg, dtype=np.float64)), ds, length,
+                        _p(np.ascontiguousarray(w, dtype=np.float64)), _p(qs))
+    return qs
+
+
+def os2_solve(laplace, ds, w, head, length):
+    shape = w.shape
+    qs = np.zeros((length,) + shape)
+    qs[0] = head
+    lib().or_os2_solve(*shape, _p(np.ascontiguousarray(laplace, dtype=np.float64)), ds, length,
+                       _p(np.ascontiguousarray(w, dtype=np.float64)), _p(qs))
+    return qs
+
+
+def quadrature(kind, q, qc, cc, ds):
+    """kind: 'simpson', 'quad4', 'trap'.  Returns None on an invalid slice count."""
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    qc = np.ascontiguousarray(qc, dtype=np.float64)
+    length = q.shape[0]
+    M = q[0].size
+    out = np.zeros(q.shape[1:])
+    f = {"simpson": lib().or_simpson, "quad4": lib().or_quad4_closed, "trap": lib().or_trapezoid}[kind]
+    rc = f(M, length, _p(q), _p(qc), cc, ds, _p(out))
+    return None if rc != 0 else out
+
+
+def lu_solve(A, b):
+    A = np.ascontiguousarray(A, dtype=np.float64).copy()
+    b = np.ascontiguousarray(b, dtype=np.float64).copy()
+    x = np.zeros_like(b)
+    rc = lib().or_lu_solve(b.size, _p(A), _p(b), _p(x))
+    return x, rc
+
+
+class Model:
+    """Model_AB (Anderson, incompressible) on the oracle."""
+
+    def __init__(self, wA0, wB0, *, dim, cell, Ms, chiN, lam=(0.1, 0.1), n_anderson=20,
+                 mde="RQM4", contour="SIMPSON"):
+        wA0 = np.ascontiguousarray(wA0, dtype=np.float64)
+        wB0 = np.ascontiguousarray(wB0, dtype=np.float64)
+        self.shape = wA0.shape
+        cs, a, b, c, al, be, ga = (list(cell) + [0.0] * 7)[:7]
+        code = CRYSTAL[cs] if isinstance(cs, str) else int(cs)
+        self._h = lib().or_model_create(dim, *self.shape, code, a, b, c, al, be, ga,
+                                        int(Ms[0]), int(Ms[1]), chiN, lam[0], lam[1],
+                                        n_anderson, 0 if mde.upper() == "RQM4" else 1,
+                                        0 if contour.upper() == "SIMPSON" else 1,
+                                        _p(wA0), _p(wB0))
+        if not self._h:
+            raise ValueError("oracle model creation failed")
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().or_model_destroy(self._h)
+            self._h = None
+
+    def update(self):
+        lib().or_model_update(self._h)
+
+    def H(self):
+        return lib().or_model_H(self._h)
+
+    def Hw(self):
+        return lib().or_model_Hw(self._h)
+
+    def Hs(self):
+        return lib().or_model_Hs(self._h)
+
+    def Q(self):
+        return lib().or_model_Q(self._h)
+
+    def residual(self):
+        return lib().or_model_residual(self._h)
+
+    def get(self, which):
+        out = np.zeros(self.shape)
+        idx = {"wA": 0, "wB": 1, "phiA": 2, "phiB": 3}[which]
+        lib().or_model_get(self._h, idx, _p(out))
+        return out
+
+    def relax(self, max_iter, min_iter=10, record_interval=1, thresh=1e-8):
+        H = np.zeros(1)
+        r = np.zeros(1)
+        it = lib().or_model_relax(self._h, max_iter, min_iter, record_interval, thresh, _p(H), _p(r))
+        return it, float(H[0]), float(r[0])
+
+
+def set_threads(n):
+    lib().or_set_num_threads(int(n))
+
+
+def num_threads():
+    return lib().or_num_threads()

@@ -1,0 +1,753 @@
+/*
+ * po_oracle.c -- CPU FP64 restatement of the PolyOrder AB/PBC SCFT hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP implementation.  Only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.
+ * Pinning status of each function is listed in po_oracle.h.
+ *
+ * The reference keeps every array C-ordered (x, y, z), z fastest (blitz
+ * default); FieldAX's internal column-major copies (src/FieldAX.cc:189-192)
+ * only change the summation order of its means, which this restatement does
+ * in C order.
+ */
+#include "po_oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define OR_PI 3.14159265358979323846264338327950288
+
+int or_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+void or_set_num_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
+/* ------------------------------------------------------------------ FFT --
+ * Mixed-radix Stockham autosort transform (Van Loan, "Computational
+ * Frameworks for the FFT", sec. 1.7).  Stage with sub-transform length L and
+ * radix R: for j in [0, n/R), k = j mod L,
+ *     v_r = x[j + r n/R] * W_{LR}^{r k}     (W_m = exp(sign 2 pi i / m))
+ *     v   = DFT_R(v)
+ *     y[(j/L) L R + k + r L] = v_r.
+ * DFT_R is evaluated from its definition, so any radix is exact.  Radices
+ * 4, 2, 3, 5 are peeled first; a remaining prime p becomes one radix-p
+ * stage. */
+
+typedef struct {
+    int n;
+    int nst;
+    int radix[32];
+    double *tw;     /* n roots exp(-2 pi i m / n), interleaved */
+    double *stw;    /* per stage: L*R twiddles W_{LR}^{r k} (forward), [k][r] */
+    double *dft;    /* per stage: R*R matrix W_R^{q r} (forward), [q][r] */
+    int stw_off[32], dft_off[32];
+} fft_plan;
+
+static void plan_init(fft_plan *p, int n) {
+    p->n = n;
+    p->nst = 0;
+    int m = n;
+    static const int pref[] = {4, 2, 3, 5};
+    for (int t = 0; t < 4; t++)
+        while (m % pref[t] == 0) { p->radix[p->nst++] = pref[t]; m /= pref[t]; }
+    for (int f = 7; m > 1; f += 2)
+        while (m % f == 0) { p->radix[p->nst++] = f; m /= f; }
+    p->tw = (double *)malloc(sizeof(double) * 2 * (n > 0 ? n : 1));
+    for (int i = 0; i < n; i++) {
+        long double ang = -2.0L * 3.141592653589793238462643383279502884L * i / n;
+        p->tw[2 * i] = (double)cosl(ang);
+        p->tw[2 * i + 1] = (double)sinl(ang);
+    }
+    int ns = 0, nd = 0, L = 1;
+    for (int st = 0; st < p->nst; st++) {
+        p->stw_off[st] = ns; p->dft_off[st] = nd;
+        ns += L * p->radix[st]; nd += p->radix[st] * p->radix[st];
+        L *= p->radix[st];
+    }
+    p->stw = (double *)malloc(sizeof(double) * 2 * (ns > 0 ? ns : 1));
+    p->dft = (double *)malloc(sizeof(double) * 2 * (nd > 0 ? nd : 1));
+    L = 1;
+    for (int st = 0; st < p->nst; st++) {
+        int R = p->radix[st];
+        double *s = p->stw + 2 * p->stw_off[st], *d = p->dft + 2 * p->dft_off[st];
+        for (int k = 0; k < L; k++)
+            for (int r = 0; r < R; r++) {
+                long e = ((long)r * k * (n / (L * R))) % n;
+                s[2 * (k * R + r)] = p->tw[2 * e];
+                s[2 * (k * R + r) + 1] = p->tw[2 * e + 1];
+            }
+        for (int q = 0; q < R; q++)
+            for (int r = 0; r < R; r++) {
+                long e = ((long)q * r * (n / R)) % n;
+                d[2 * (q * R + r)] = p->tw[2 * e];
+                d[2 * (q * R + r) + 1] = p->tw[2 * e + 1];
+            }
+        L *= R;
+    }
+}
+
+static void plan_free(fft_plan *p) { free(p->tw); free(p->stw); free(p->dft); }
+
+/* one 1-D transform of x (interleaved, length n) using work buffer y.
+ * The backward transform uses the conjugate roots. */
+static void fft_exec(const fft_plan *p, double *x, double *y, int sign) {
+    int n = p->n;
+    if (n <= 1) return;
+    double *src = x, *dst = y;
+    int L = 1;
+    const double cs = sign < 0 ? 1.0 : -1.0;
+    double vr[64], vi[64];
+    for (int st = 0; st < p->nst; st++) {
+        int R = p->radix[st];
+        int nR = n / R;
+        const double *tw = p->stw + 2 * p->stw_off[st];
+        const double *dm = p->dft + 2 * p->dft_off[st];
+        for (int j = 0; j < nR; j++) {
+            int k = j % L;
+            const double *t = tw + 2 * k * R;
+            for (int r = 0; r < R; r++) {
+                double a = src[2 * (j + r * nR)], b = src[2 * (j + r * nR) + 1];
+                double cr = t[2 * r], ci = cs * t[2 * r + 1];
+                vr[r] = a * cr - b * ci;
+                vi[r] = a * ci + b * cr;
+            }
+            int base = (j / L) * L * R + k;
+            for (int q = 0; q < R; q++) {
+                double sr = 0.0, si = 0.0;
+                const double *row = dm + 2 * q * R;
+                for (int r = 0; r < R; r++) {
+                    double cr = row[2 * r], ci = cs * row[2 * r + 1];
+                    sr += vr[r] * cr - vi[r] * ci;
+                    si += vr[r] * ci + vi[r] * cr;
+                }
+                dst[2 * (base + q * L)] = sr;
+                dst[2 * (base + q * L) + 1] = si;
+            }
+        }
+        double *tt = src; src = dst; dst = tt;
+        L *= R;
+    }
+    if (src != x) memcpy(x, src, sizeof(double) * 2 * n);
+}
+
+/* plans are cached per length (the oracle is used from one host thread). */
+static fft_plan *plan_get(int n) {
+    static fft_plan cache[16];
+    static int ncache = 0;
+    for (int i = 0; i < ncache; i++)
+        if (cache[i].n == n) return &cache[i];
+    if (ncache == 16) { plan_free(&cache[15]); ncache = 15; }
+    plan_init(&cache[ncache], n);
+    return &cache[ncache++];
+}
+
+void or_fft1d(int n, double *data, int sign) {
+    const fft_plan *p = plan_get(n);
+    double *y = (double *)malloc(sizeof(double) * 2 * n);
+    fft_exec(p, data, y, sign);
+    free(y);
+}
+
+/* transform every pencil along one axis of a C-order (n0,n1,n2) array. */
+static void fft_axis(double *data, int n0, int n1, int n2, int axis, int sign) {
+    int len = axis == 0 ? n0 : (axis == 1 ? n1 : n2);
+    if (len <= 1) return;
+    long stride = axis == 0 ? (long)n1 * n2 : (axis == 1 ? n2 : 1);
+    long npen = (long)n0 * n1 * n2 / len;
+    const fft_plan *p = plan_get(len);
+#pragma omp parallel if (npen * len > 16384)
+    {
+        double *x = (double *)malloc(sizeof(double) * 2 * len);
+        double *y = (double *)malloc(sizeof(double) * 2 * len);
+#pragma omp for schedule(static)
+        for (long pen = 0; pen < npen; pen++) {
+            long base;
+            if (axis == 2) base = pen * n2;
+            else if (axis == 1) base = (pen / n2) * (long)n1 * n2 + (pen % n2);
+            else base = pen;
+            for (int i = 0; i < len; i++) {
+                x[2 * i] = data[2 * (base + i * stride)];
+                x[2 * i + 1] = data[2 * (base + i * stride) + 1];
+            }
+            fft_exec(p, x, y, sign);
+            for (int i = 0; i < len; i++) {
+                data[2 * (base + i * stride)] = x[2 * i];
+                data[2 * (base + i * stride) + 1] = x[2 * i + 1];
+            }
+        }
+        free(x);
+        free(y);
+    }
+}
+
+void or_fft3d(int Lx, int Ly, int Lz, double *data, int sign) {
+    fft_axis(data, Lx, Ly, Lz, 2, sign);
+    fft_axis(data, Lx, Ly, Lz, 1, sign);
+    fft_axis(data, Lx, Ly, Lz, 0, sign);
+}
+
+/* ------------------------------------------------------------- UnitCell --
+ * Crystal-system codes follow the order of CrystalSystemType
+ * (include/common.h:149-163). */
+enum {
+    CS_LAMELLAR = 0, CS_SQUARE, CS_RECTANGULAR, CS_HEXAGONAL, CS_OBLIQUE,
+    CS_CUBIC, CS_TETRAGONAL, CS_ORTHORHOMBIC, CS_TRIGONAL, CS_MONOCLINIC,
+    CS_TRICLINIC
+};
+
+static void cross3(const double *u, const double *v, double *o) {
+    o[0] = u[1] * v[2] - u[2] * v[1];
+    o[1] = u[2] * v[0] - u[0] * v[2];
+    o[2] = u[0] * v[1] - u[1] * v[0];
+}
+static double dot3(const double *u, const double *v) {
+    return u[0] * v[0] + u[1] * v[1] + u[2] * v[2];
+}
+
+int or_cell_shape(int dim, int cs, double a, double b, double c,
+                  double alpha, double beta, double gamma,
+                  double *l, double *g) {
+    const double pi2 = OR_PI / 2.0, pi3 = 2.0 * OR_PI / 3.0, twopi = 2.0 * OR_PI;
+    memset(g, 0, sizeof(double) * 9);
+    if (dim == 1) { /* calc_shape_matrix1, src/UnitCell.cc:142-154 */
+        l[0] = a; l[1] = 0.0; l[2] = 0.0;
+        g[0] = twopi / a; g[4] = twopi; g[8] = twopi;
+        return 0;
+    }
+    if (dim == 2) { /* calc_shape_matrix2, src/UnitCell.cc:156-222 */
+        if (a <= 0) return -1;
+        switch (cs) {
+        case CS_SQUARE: b = a; gamma = pi2; break;
+        case CS_RECTANGULAR: if (b <= 0) return -1; gamma = pi2; break;
+        case CS_HEXAGONAL: b = a; gamma = pi3; break;
+        default: if (b <= 0 || gamma <= 0) return -1; break;
+        }
+        double A[3] = {a, 0, 0}, B[3] = {b * cos(gamma), b * sin(gamma), 0};
+        l[0] = a; l[1] = b; l[2] = 0.0;
+        double acb[3], as[3], bs[3];
+        cross3(A, B, acb);
+        double denom = dot3(acb, acb);
+        cross3(B, acb, as);
+        cross3(acb, A, bs);
+        for (int i = 0; i < 3; i++) { as[i] /= denom; bs[i] /= denom; }
+        g[0] = twopi * as[0]; g[1] = twopi * bs[0]; g[2] = 0;
+        g[3] = twopi * as[1]; g[4] = twopi * bs[1]; g[5] = 0;
+        g[6] = 0; g[7] = 0; g[8] = twopi;
+        return 0;
+    }
+    /* calc_shape_matrix3, src/UnitCell.cc:224-296 */
+    switch (cs) {
+    case CS_CUBIC: b = a; c = a; alpha = beta = gamma = pi2; break;
+    case CS_TETRAGONAL: b = a; alpha = beta = gamma = pi2; break;
+    case CS_ORTHORHOMBIC: alpha = beta = gamma = pi2; break;
+    case CS_HEXAGONAL: b = a; alpha = beta = pi2; gamma = pi3; break;
+    case CS_TRIGONAL: b = a; c = a; alpha = gamma; beta = gamma; break;
+    case CS_MONOCLINIC: alpha = pi2; gamma = pi2; break;
+    default: return -1;
+    }
+    double A[3] = {a, 0, 0};
+    double B[3] = {b * cos(gamma), b * sin(gamma), 0};
+    double cx = c * cos(beta);
+    double cy = c * (cos(alpha) - cos(beta) * cos(gamma));
+    double cz = sqrt(c * c - cx * cx - cy * cy);
+    double C[3] = {cx, cy, cz};
+    l[0] = a; l[1] = b; l[2] = c;
+    double bcc[3], cca[3], acb[3];
+    cross3(B, C, bcc);
+    cross3(C, A, cca);
+    cross3(A, B, acb);
+    double denom = dot3(A, bcc);
+    for (int i = 0; i < 3; i++) {
+        g[3 * i + 0] = twopi * bcc[i] / denom;
+        g[3 * i + 1] = twopi * cca[i] / denom;
+        g[3 * i + 2] = twopi * acb[i] / denom;
+    }
+    return 0;
+}
+
+void or_k2_orthogonal(int dim, const double *l, int Lx, int Ly, int Lz,
+                      double *k2) {
+    /* src/UnitCell.cc:368-418; the sign is carried by ccx, ccy, ccz. */
+    double ccx = 0.0, ccy = 0.0, ccz = 0.0;
+    if (dim >= 1) ccx = -4.0 * OR_PI * OR_PI / (l[0] * l[0]);
+    if (dim >= 2) ccy = -4.0 * OR_PI * OR_PI / (l[1] * l[1]);
+    if (dim >= 3) ccz = -4.0 * OR_PI * OR_PI / (l[2] * l[2]);
+    for (int i = 0; i < Lx; i++)
+        for (int j = 0; j < Ly; j++)
+            for (int k = 0; k < Lz; k++) {
+                int ri = i < Lx / 2 + 1 ? i : Lx - i;
+                int rj = j < Ly / 2 + 1 ? j : Ly - j;
+                int rk = k < Lz / 2 + 1 ? k : Lz - k;
+                k2[((long)i * Ly + j) * Lz + k] =
+                    ccx * ri * ri + ccy * rj * rj + ccz * rk * rk;
+            }
+}
+
+void or_klaplacian(const double *g, int Lx, int Ly, int Lz, double ds,
+                   double *out) {
+    /* src/UnitCell.cc:321-359 */
+    int N[3] = {Lx, Ly, Lz};
+    for (int i = 0; i < Lx; i++)
+        for (int j = 0; j < Ly; j++)
+            for (int k = 0; k < Lz; k++) {
+                double q2min = 1e99;
+                for (int a = 1; a > -2; a--)
+                    for (int b = 1; b > -2; b--)
+                        for (int c = 1; c > -2; c--) {
+                            double bq[3] = {(double)(a * N[0]) + i,
+                                            (double)(b * N[1]) + j,
+                                            (double)(c * N[2]) + k};
+                            double qt[3];
+                            for (int r = 0; r < 3; r++)
+                                qt[r] = g[3 * r] * bq[0] + g[3 * r + 1] * bq[1] +
+                                        g[3 * r + 2] * bq[2];
+                            double q2 = dot3(qt, qt);
+                            if (q2 < q2min) q2min = q2;
+                        }
+                out[((long)i * Ly + j) * Lz + k] = exp(-ds * q2min);
+            }
+}
+
+/* ----------------------------------------------------------- propagators */
+
+/* one split step: out = (e * F^-1[ lap * F(e * in) ]) / M, the body shared
+ * by src/RQM4.cc:71-75,79-89 and src/PseudoSpectral.cc:95-102. */
+static void split_step(int Lx, int Ly, int Lz, const double *lap,
+                       const double *e, const double *in, double *out,
+                       double *buf) {
+    long M = (long)Lx * Ly * Lz;
+#pragma omp parallel for schedule(static) if (M > 16384)
+    for (long i = 0; i < M; i++) {
+        buf[2 * i] = e[i] * in[i];
+        buf[2 * i + 1] = 0.0;
+    }
+    or_fft3d(Lx, Ly, Lz, buf, -1);
+#pragma omp parallel for schedule(static) if (M > 16384)
+    for (long i = 0; i < M; i++) {
+        buf[2 * i] *= lap[i];
+        buf[2 * i + 1] *= lap[i];
+    }
+    or_fft3d(Lx, Ly, Lz, buf, +1);
+#pragma omp parallel for schedule(static) if (M > 16384)
+    for (long i = 0; i < M; i++) out[i] = (buf[2 * i] * e[i]) / (double)M;
+}
+
+void or_rqm4_solve(int Lx, int Ly, int Lz, const double *k2neg, double ds,
+                   int len, const double *w, double *qs) {
+    long M = (long)Lx * Ly * Lz;
+    double *lap = (double *)malloc(sizeof(double) * M);
+    double *lap2 = (double *)malloc(sizeof(double) * M);
+    double *e = (double *)malloc(sizeof(double) * M);
+    double *e2 = (double *)malloc(sizeof(double) * M);
+    double *qt = (double *)malloc(sizeof(double) * M);
+    double *buf = (double *)malloc(sizeof(double) * 2 * M);
+    for (long i = 0; i < M; i++) {
+        lap[i] = exp(ds * k2neg[i]);        /* src/RQM4.cc:13 */
+        lap2[i] = exp(0.5 * ds * k2neg[i]); /* src/RQM4.cc:14 */
+        e[i] = exp(-0.5 * ds * w[i]);       /* src/RQM4.cc:50 */
+        e2[i] = exp(-0.25 * ds * w[i]);     /* src/RQM4.cc:52 */
+    }
+    for (int s = 0; s < len - 1; s++) {     /* src/RQM4.cc:66-92 */
+        const double *q1 = qs + (long)s * M;
+        double *q2 = qs + (long)(s + 1) * M;
+        split_step(Lx, Ly, Lz, lap, e, q1, q2, buf);
+        split_step(Lx, Ly, Lz, lap2, e2, q1, qt, buf);
+        split_step(Lx, Ly, Lz, lap2, e2, qt, qt, buf);
+#pragma omp parallel for schedule(static) if (M > 16384)
+        for (long i = 0; i < M; i++) q2[i] = (4.0 * qt[i] - q2[i]) / 3.0;
+    }
+    free(lap); free(lap2); free(e); free(e2); free(qt); free(buf);
+}
+
+void or_os2_solve(int Lx, int Ly, int Lz, const double *laplace, double ds,
+                  int len, const double *w, double *qs) {
+    long M = (long)Lx * Ly * Lz;
+    double *e = (double *)malloc(sizeof(double) * M);
+    double *buf = (double *)malloc(sizeof(double) * 2 * M);
+    for (long i = 0; i < M; i++) e[i] = exp(-0.5 * ds * w[i]); /* :78 */
+    for (int s = 0; s < len - 1; s++)                          /* :91-103 */
+        split_step(Lx, Ly, Lz, laplace, e, qs + (long)s * M,
+                   qs + (long)(s + 1) * M, buf);
+    free(e); free(buf);
+}
+
+/* ----------------------------------------------------------- quadrature */
+
+int or_simpson(int M, int len, const double *q, const double *qc, double cc,
+               double ds, double *out) {
+    int n = len - 1; /* src/Simpson.cc:18-26 */
+    if (n % 2 != 0 || n < 4) return -1;
+#pragma omp parallel for schedule(static) if (M > 16384)
+    for (long i = 0; i < M; i++) {
+        double s0 = q[i] * qc[(long)n * M + i] + q[(long)n * M + i] * qc[i];
+        double s1 = 0.0, s2 = 0.0;
+        for (int s = 1; s <= n - 1; s += 2)
+            s1 += 4.0 * (q[(long)s * M + i] * qc[(long)(n - s) * M + i]);
+        for (int s = 2; s <= n - 2; s += 2)
+            s2 += 2.0 * (q[(long)s * M + i] * qc[(long)(n - s) * M + i]);
+        out[i] = (s0 + s1 + s2) * (cc * ds / 3.0); /* :42-44 */
+    }
+    return 0;
+}
+
+int or_quad4_closed(int M, int len, const double *q, const double *qc,
+                    double cc, double ds, double *out) {
+    int n = len - 1; /* src/Quadrature4.cc:21-52 */
+    if (n < 7) return -1;
+    const double c0 = 0.375, c1 = 7 / 6.0, c2 = 23 / 24.0;
+#pragma omp parallel for schedule(static) if (M > 16384)
+    for (long i = 0; i < M; i++) {
+#define QQ(s) (q[(long)(s) * M + i] * qc[(long)(n - (s)) * M + i])
+        double v = c0 * QQ(0) + c1 * QQ(1) + c2 * QQ(2);
+        double mid = 0.0;
+        for (int s = 3; s <= n - 3; s++) mid += QQ(s);
+        v += mid;
+        v += c2 * QQ(n - 2) + c1 * QQ(n - 1) + c0 * QQ(n);
+#undef QQ
+        out[i] = v * (cc * ds);
+    }
+    return 0;
+}
+
+int or_trapezoid(int M, int len, const double *q, const double *qc, double cc,
+                 double ds, double *out) {
+    int n = len - 1; /* src/Density.cc:107-121 */
+    if (n < 1) return -1;
+#pragma omp parallel for schedule(static) if (M > 16384)
+    for (long i = 0; i < M; i++) {
+        double sum = 0.0;
+        for (int s = 0; s <= n; s++)
+            sum += q[(long)s * M + i] * qc[(long)(n - s) * M + i];
+        double q0 = 0.5 * q[i] * qc[(long)n * M + i];
+        double qt = 0.5 * qc[i] * q[(long)n * M + i];
+        out[i] = (sum - (q0 + qt)) * (cc * ds);
+    }
+    return 0;
+}
+
+double or_mean(int M, const double *x) {
+    double s = 0.0; /* blitz::mean: sequential sum / size */
+    for (long i = 0; i < M; i++) s += x[i];
+    return s / M;
+}
+
+int or_lu_solve(int n, double *A, double *b, double *x) {
+    /* Gaussian elimination with partial pivoting (dgesv semantics). */
+    int *piv = (int *)malloc(sizeof(int) * (n > 0 ? n : 1));
+    int rc = 0;
+    for (int i = 0; i < n; i++) piv[i] = i;
+    for (int k = 0; k < n; k++) {
+        int p = k;
+        double best = fabs(A[k * n + k]);
+        for (int i = k + 1; i < n; i++)
+            if (fabs(A[i * n + k]) > best) { best = fabs(A[i * n + k]); p = i; }
+        if (best == 0.0) { rc = -1; break; }
+        if (p != k) {
+            for (int j = 0; j < n; j++) {
+                double t = A[k * n + j]; A[k * n + j] = A[p * n + j]; A[p * n + j] = t;
+            }
+            double t = b[k]; b[k] = b[p]; b[p] = t;
+        }
+        for (int i = k + 1; i < n; i++) {
+            double f = A[i * n + k] / A[k * n + k];
+            A[i * n + k] = f;
+            for (int j = k + 1; j < n; j++) A[i * n + j] -= f * A[k * n + j];
+            b[i] -= f * b[k];
+        }
+    }
+    if (rc == 0) {
+        for (int i = n - 1; i >= 0; i--) {
+            double s = b[i];
+            for (int j = i + 1; j < n; j++) s -= A[i * n + j] * x[j];
+            x[i] = s / A[i * n + i];
+        }
+    } else {
+        for (int i = 0; i < n; i++) x[i] = 0.0;
+    }
+    free(piv);
+    return rc;
+}
+
+/* -------------------------------------------------------------- FieldAX --
+ * Ring buffer of n+1 fields and residuals (src/FieldAX.cc:11-39,
+ * include/FieldAX.h:99-179).  cur_pos starts at n with w(n) = w0. */
+typedef struct {
+    int n, M;
+    int cur_pos;
+    long k;
+    double lambda;
+    double *w, *d; /* (n+1) * M each */
+    double *B;     /* (n+1)^2 row-major */
+    double *data;  /* Grid::_data: the current field */
+} fieldax;
+
+static void fax_init(fieldax *f, int M, int n, double lam, const double *w0) {
+    f->n = n; f->M = M; f->cur_pos = n; f->k = 0; f->lambda = lam;
+    f->w = (double *)calloc((size_t)(n + 1) * M, sizeof(double));
+    f->d = (double *)calloc((size_t)(n + 1) * M, sizeof(double));
+    f->B = (double *)calloc((size_t)(n + 1) * (n + 1), sizeof(double));
+    f->data = (double *)malloc(sizeof(double) * M);
+    memcpy(f->data, w0, sizeof(double) * M);
+    memcpy(f->w + (long)n * M, w0, sizeof(double) * M); /* update_w0 :274-280 */
+}
+
+static void fax_free(fieldax *f) { free(f->w); free(f->d); free(f->B); free(f->data); }
+
+static int fax_find_index(const fieldax *f, int j) { /* :262-267 */
+    int i = j + f->cur_pos;
+    if (i > f->n) i -= f->n + 1;
+    return i;
+}
+
+static int fax_nn(const fieldax *f) { return f->k < f->n ? (int)f->k : f->n; }
+
+/* pre_update = update_d + update_B (src/FieldAX.cc:188-199,311-345) */
+static void fax_pre_update(fieldax *f, const double *wt, double *tmp) {
+    int n = f->n, M = f->M;
+    double *dc = f->d + (long)f->cur_pos * M;
+    const double *wc = f->w + (long)f->cur_pos * M;
+    for (long i = 0; i < M; i++) dc[i] = wt[i] - wc[i];
+    for (int i = n; i >= 1; i--)
+        for (int j = n; j >= i; j--)
+            f->B[i * (n + 1) + j] = f->B[(i - 1) * (n + 1) + (j - 1)];
+    int nn = fax_nn(f);
+    for (int j = 0; j <= nn; j++) {
+        const double *dk = f->d + (long)fax_find_index(f, j) * M;
+        for (long i = 0; i < M; i++) tmp[i] = dc[i] * dk[i];
+        f->B[j] = or_mean(M, tmp);
+    }
+    for (int i = 0; i <= n; i++) /* arma::symmatu */
+        for (int j = i + 1; j <= n; j++) f->B[j * (n + 1) + i] = f->B[i * (n + 1) + j];
+}
+
+static void fax_calc_UV(const fieldax *f, double *U, double *V) { /* :352-381 */
+    int n1 = f->n + 1, nn = fax_nn(f);
+    const double *B = f->B;
+    for (int i = 0; i < nn; i++) {
+        for (int j = 0; j < nn; j++)
+            U[i * nn + j] = B[0] - B[(i + 1) * n1] - B[j + 1] + B[(i + 1) * n1 + (j + 1)];
+        V[i] = B[0] - B[(i + 1) * n1];
+    }
+}
+
+/* update(C) = calc_W, calc_D, rotate cur_pos, update_w (:201-223,286-305) */
+static void fax_update(fieldax *f, const double *C, int nc, double *W, double *D) {
+    int M = f->M;
+    const double *wc = f->w + (long)f->cur_pos * M;
+    const double *dc = f->d + (long)f->cur_pos * M;
+    memcpy(W, wc, sizeof(double) * M);
+    memcpy(D, dc, sizeof(double) * M);
+    for (int i = 1; i <= nc; i++) {
+        int kk = fax_find_index(f, i);
+        const double *wk = f->w + (long)kk * M, *dk = f->d + (long)kk * M;
+        for (long r = 0; r < M; r++) {
+            W[r] += C[i - 1] * (wk[r] - wc[r]);
+            D[r] += C[i - 1] * (dk[r] - dc[r]);
+        }
+    }
+    f->cur_pos = f->cur_pos > 0 ? f->cur_pos - 1 : f->n;
+    double lam = 1.0 - pow(f->lambda, (double)(f->k + 1));
+    double *wn = f->w + (long)f->cur_pos * M;
+    for (long r = 0; r < M; r++) wn[r] = W[r] + lam * D[r];
+    memcpy(f->data, wn, sizeof(double) * M);
+    f->k++;
+}
+
+/* current_{d2,w2}_quadrature read ring slot cur_pos+1 (:106-116,140-150) */
+static double fax_d2(const fieldax *f, double *tmp) {
+    int pos = f->cur_pos + 1 > f->n ? 0 : f->cur_pos + 1;
+    const double *x = f->d + (long)pos * f->M;
+    for (long i = 0; i < f->M; i++) tmp[i] = x[i] * x[i];
+    return or_mean(f->M, tmp);
+}
+static double fax_w2(const fieldax *f, double *tmp) {
+    int pos = f->cur_pos + 1 > f->n ? 0 : f->cur_pos + 1;
+    const double *x = f->w + (long)pos * f->M;
+    for (long i = 0; i < f->M; i++) tmp[i] = x[i] * x[i];
+    return or_mean(f->M, tmp);
+}
+
+/* ------------------------------------------------------------- Model_AB */
+struct or_model {
+    int dim, Lx, Ly, Lz, M;
+    int sA, sB, mde, contour;
+    double chiN, fA, fB, dsA, dsB, Q;
+    double l[3], g[9];
+    double *k2neg;          /* RQM4 table (calc_k2_orthogonal) */
+    double *lapA, *lapB;    /* OS2 tables (calc_kLaplacian) */
+    double *qA, *qB, *qAc, *qBc;
+    double *phiA, *phiB;
+    double *tmp, *tmp2, *W, *D;
+    fieldax wA, wB;
+};
+
+or_model *or_model_create(int dim, int Lx, int Ly, int Lz,
+                          int cs, double a, double b, double c,
+                          double alpha, double beta, double gamma,
+                          int sA, int sB, double chiN,
+                          double lamA, double lamB, int n_anderson,
+                          int mde, int contour,
+                          const double *wA0, const double *wB0) {
+    or_model *m = (or_model *)calloc(1, sizeof(or_model));
+    m->dim = dim; m->Lx = Lx; m->Ly = Ly; m->Lz = Lz;
+    long M = (long)Lx * Ly * Lz;
+    m->M = (int)M;
+    m->sA = sA; m->sB = sB; m->mde = mde; m->contour = contour; m->chiN = chiN;
+    /* src/Model_AB.cc:100-111 */
+    int LA = sA - 1, LB = sB - 1;
+    m->fA = 1.0 * LA / (LA + LB);
+    m->fB = 1.0 * LB / (LA + LB);
+    m->dsA = m->fA / LA;
+    m->dsB = m->fB / LB;
+    if (or_cell_shape(dim, cs, a, b, c, alpha, beta, gamma, m->l, m->g) != 0) {
+        free(m);
+        return NULL;
+    }
+    if (mde == 0) {
+        m->k2neg = (double *)malloc(sizeof(double) * M);
+        or_k2_orthogonal(dim, m->l, Lx, Ly, Lz, m->k2neg);
+    } else {
+        m->lapA = (double *)malloc(sizeof(double) * M);
+        m->lapB = (double *)malloc(sizeof(double) * M);
+        or_klaplacian(m->g, Lx, Ly, Lz, m->dsA, m->lapA);
+        or_klaplacian(m->g, Lx, Ly, Lz, m->dsB, m->lapB);
+    }
+    m->qA = (double *)calloc((size_t)sA * M, sizeof(double));
+    m->qAc = (double *)calloc((size_t)sA * M, sizeof(double));
+    m->qB = (double *)calloc((size_t)sB * M, sizeof(double));
+    m->qBc = (double *)calloc((size_t)sB * M, sizeof(double));
+    for (long i = 0; i < M; i++) { m->qA[i] = 1.0; m->qBc[i] = 1.0; } /* :779-782 */
+    m->phiA = (double *)calloc(M, sizeof(double));
+    m->phiB = (double *)calloc(M, sizeof(double));
+    m->tmp = (double *)malloc(sizeof(double) * M);
+    m->tmp2 = (double *)malloc(sizeof(double) * M);
+    m->W = (double *)malloc(sizeof(double) * M);
+    m->D = (double *)malloc(sizeof(double) * M);
+    fax_init(&m->wA, (int)M, n_anderson, lamA, wA0);
+    fax_init(&m->wB, (int)M, n_anderson, lamB, wB0);
+    return m;
+}
+
+void or_model_destroy(or_model *m) {
+    if (!m) return;
+    free(m->k2neg); free(m->lapA); free(m->lapB);
+    free(m->qA); free(m->qB); free(m->qAc); free(m->qBc);
+    free(m->phiA); free(m->phiB); free(m->tmp); free(m->tmp2); free(m->W); free(m->D);
+    fax_free(&m->wA); fax_free(&m->wB);
+    free(m);
+}
+
+static void solve_prop(or_model *m, double *qs, int len, double ds,
+                       const double *lap, const double *w) {
+    if (m->mde == 0) or_rqm4_solve(m->Lx, m->Ly, m->Lz, m->k2neg, ds, len, w, qs);
+    else or_os2_solve(m->Lx, m->Ly, m->Lz, lap, ds, len, w, qs);
+}
+
+static void density(or_model *m, int len, const double *q, const double *qc,
+                    double cc, double ds, double *out) {
+    /* src/Model_AB.cc:713-735 quadrature selection */
+    if (m->contour == 1) or_trapezoid(m->M, len, q, qc, cc, ds, out);
+    else if (len % 2 != 0) or_simpson(m->M, len, q, qc, cc, ds, out);
+    else or_quad4_closed(m->M, len, q, qc, cc, ds, out);
+}
+
+void or_model_update(or_model *m) {
+    long M = m->M;
+    const double *wA = m->wA.data, *wB = m->wB.data;
+    /* src/Model_AB.cc:139-145 */
+    solve_prop(m, m->qA, m->sA, m->dsA, m->lapA, wA);
+    memcpy(m->qB, m->qA + (long)(m->sA - 1) * M, sizeof(double) * M);
+    solve_prop(m, m->qB, m->sB, m->dsB, m->lapB, wB);
+    solve_prop(m, m->qBc, m->sB, m->dsB, m->lapB, wB);
+    memcpy(m->qAc, m->qBc + (long)(m->sB - 1) * M, sizeof(double) * M);
+    solve_prop(m, m->qAc, m->sA, m->dsA, m->lapA, wA);
+    /* :195-199 */
+    m->Q = or_mean((int)M, m->qB + (long)(m->sB - 1) * M);
+    density(m, m->sA, m->qA, m->qAc, 1.0 / m->Q, m->dsA, m->phiA);
+    density(m, m->sB, m->qB, m->qBc, 1.0 / m->Q, m->dsB, m->phiB);
+    /* :231-253, incompressible Anderson branch */
+    double *wtA = m->W, *wtB = m->D;
+    for (long i = 0; i < M; i++) {
+        double yita = 0.5 * (wA[i] + wB[i] - m->chiN);
+        wtA[i] = m->chiN * m->phiB[i] + yita;
+        wtB[i] = m->chiN * m->phiA[i] + yita;
+    }
+    fax_pre_update(&m->wA, wtA, m->tmp);
+    fax_pre_update(&m->wB, wtB, m->tmp);
+    int nn = fax_nn(&m->wA);
+    double *U = (double *)calloc((size_t)nn * nn + 1, sizeof(double));
+    double *V = (double *)calloc((size_t)nn + 1, sizeof(double));
+    double *U2 = (double *)calloc((size_t)nn * nn + 1, sizeof(double));
+    double *V2 = (double *)calloc((size_t)nn + 1, sizeof(double));
+    double *C = (double *)calloc((size_t)nn + 1, sizeof(double));
+    fax_calc_UV(&m->wA, U, V);
+    fax_calc_UV(&m->wB, U2, V2);
+    for (int i = 0; i < nn * nn; i++) U[i] += U2[i];
+    for (int i = 0; i < nn; i++) V[i] += V2[i];
+    if (nn > 0) or_lu_solve(nn, U, V, C);
+    fax_update(&m->wA, C, nn, m->W, m->D);
+    fax_update(&m->wB, C, nn, m->W, m->D);
+    free(U); free(V); free(U2); free(V2); free(C);
+}
+
+double or_model_Hw(const or_model *m) {
+    /* src/Model_AB.cc:280-284: uses the fields as they are after update() */
+    for (long i = 0; i < m->M; i++)
+        m->tmp[i] = m->chiN * m->phiA[i] * m->phiB[i] - m->wA.data[i] * m->phiA[i] -
+                    m->wB.data[i] * m->phiB[i];
+    return or_mean(m->M, m->tmp);
+}
+
+double or_model_Hs(const or_model *m) { return -log(m->Q); } /* :294-296 */
+double or_model_H(const or_model *m) { return or_model_Hw(m) + or_model_Hs(m); }
+double or_model_Q(const or_model *m) { return m->Q; }
+
+double or_model_residual(const or_model *m) {
+    /* src/Model_AB.cc:309-315 */
+    double eA1 = fax_d2(&m->wA, m->tmp), eA2 = fax_w2(&m->wA, m->tmp);
+    double eB1 = fax_d2(&m->wB, m->tmp), eB2 = fax_w2(&m->wB, m->tmp);
+    return sqrt((eA1 + eB1) / (eA2 + eB2));
+}
+
+void or_model_get(const or_model *m, int which, double *out) {
+    const double *src = which == 0 ? m->wA.data : which == 1 ? m->wB.data
+                      : which == 2 ? m->phiA : m->phiB;
+    memcpy(out, src, sizeof(double) * m->M);
+}
+
+int or_model_relax(or_model *m, int max_iter, int min_iter,
+                   int record_interval, double thresh,
+                   double *H_out, double *res_out) {
+    int i_save = -1, it;
+    double H = 0.0, res = 0.0;
+    for (it = 1; it <= max_iter; it++) { /* src/scft.cc:395-421 */
+        or_model_update(m);
+        if (it % record_interval == 0) {
+            i_save++;
+            H = or_model_H(m);
+            res = or_model_residual(m);
+        }
+        if (i_save >= 0 && !isfinite(H)) break;               /* :537-543 */
+        if (it > min_iter && i_save > 0 && res < thresh) break; /* :560-568 */
+    }
+    if (it > max_iter) it = max_iter;
+    if (H_out) *H_out = H;
+    if (res_out) *res_out = res;
+    return it;
+}

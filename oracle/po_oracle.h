@@ -1,0 +1,134 @@
+/*
+ * po_oracle.h -- CPU FP64 restatement of PolyOrder's periodic pseudospectral
+ * SCFT hot path (AB diblock melt, RQM4 / OS2 propagators, Simpson / Quad4 /
+ * trapezoid contour quadrature, Anderson mixing).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP
+ * implementation in polystring_amd/: only tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg may load it.  The product never calls it.
+ *
+ * Every function cites the reference file:line it restates.  The FFT is the
+ * project's own mixed-radix Stockham C2C transform standing in for FFTW 3.3+
+ * (fftw_plan_dft_3d, FFTW_ESTIMATE; src/RQM4.cc:28-31); it is pinned against
+ * a naive DFT and numpy.fft, not against FFTW's own rounding.
+ *
+ * Pinning status per function (what fails if a plausible mistake is made):
+ *   or_fft1d / or_fft3d  : naive DFT (tests) and numpy.fft golden vectors.
+ *   or_k2_orthogonal     : closed form 4 pi^2 r^2 / l^2 with r=min(i,L-i);
+ *                          cross-checked against or_klaplacian (27-image min,
+ *                          tests/test_unitcell.cc:56-70 equality check).
+ *   or_rqm4_solve/os2    : single Fourier mode exp(-k^2 s); constant field
+ *                          exp(-c s); 4th- vs 2nd-order convergence; adjoint
+ *                          symmetry <q(s) qc(N-s)> = Q; published H values.
+ *   or_simpson/quad4/trap: closed form exp(fA)-1 (tests/test_quadrature4.cc).
+ *   or_model_*           : published converged H (polyorder-master example
+ *                          LAM 1D/2D/3D param.ini headers), <phiA>=fA.
+ *   Anderson LU fallback on an exactly singular U: parity unpinned
+ *   (arma::solve behaviour is version dependent, src/FieldAX.cc:386-390).
+ */
+#ifndef PO_ORACLE_H
+#define PO_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* 1-D complex DFT of length n, interleaved (re,im), in place.
+ * sign = -1 forward (FFTW_FORWARD), +1 backward; both unnormalised. */
+void or_fft1d(int n, double *data, int sign);
+
+/* 3-D C2C DFT over a C-order (Lx,Ly,Lz) interleaved complex array, in place,
+ * unnormalised (fftw_plan_dft_3d, src/RQM4.cc:28-31). */
+void or_fft3d(int Lx, int Ly, int Lz, double *data, int sign);
+
+/* UnitCell::calc_shape_matrix{1,2,3} (src/UnitCell.cc:142-296).
+ * cs: crystal system code (see po_oracle.c).  Returns 0 on success.
+ * l[3] = cell lengths, g[9] = reciprocal shape matrix, row-major. */
+int or_cell_shape(int dim, int cs, double a, double b, double c,
+                  double alpha, double beta, double gamma,
+                  double *l, double *g);
+
+/* UnitCell::calc_k2_orthogonal (src/UnitCell.cc:368-418): writes -|k|^2. */
+void or_k2_orthogonal(int dim, const double *l, int Lx, int Ly, int Lz,
+                      double *k2neg);
+
+/* UnitCell::calc_kLaplacian (src/UnitCell.cc:321-359):
+ * exp(-ds * min over 27 images |g (n + shift)|^2). */
+void or_klaplacian(const double *g, int Lx, int Ly, int Lz, double ds,
+                   double *out);
+
+/* RQM4::solve (src/RQM4.cc:41-93).  qs holds len slices of M doubles with
+ * slice 0 (the head) already set; slices 1..len-1 are written.
+ * k2neg = -|k|^2 table (or_k2_orthogonal); tables exp(ds k2neg) and
+ * exp(ds/2 k2neg) are built as in src/RQM4.cc:9-16. */
+void or_rqm4_solve(int Lx, int Ly, int Lz, const double *k2neg, double ds,
+                   int len, const double *w, double *qs);
+
+/* PseudoSpectral::solve (src/PseudoSpectral.cc:69-104); laplace is the
+ * exp(-ds |k|^2) table of src/PseudoSpectral.cc:33-38. */
+void or_os2_solve(int Lx, int Ly, int Lz, const double *laplace, double ds,
+                  int len, const double *w, double *qs);
+
+/* Contour quadratures: src/Simpson.cc:14-45, src/Quadrature4.cc:17-53,
+ * src/Density.cc:107-121.  Return 0 on success, -1 if the slice count is
+ * invalid (data untouched, as the reference prints and returns). */
+int or_simpson(int M, int len, const double *q, const double *qc, double cc,
+               double ds, double *out);
+int or_quad4_closed(int M, int len, const double *q, const double *qc,
+                    double cc, double ds, double *out);
+int or_trapezoid(int M, int len, const double *q, const double *qc, double cc,
+                 double ds, double *out);
+
+/* Grid::quadrature on a regular grid = blitz::mean (src/Grid.cc:329-338). */
+double or_mean(int M, const double *x);
+
+/* Dense solve A x = b, LU with partial pivoting (arma::solve -> LAPACK
+ * dgesv, src/FieldAX.cc:386-390).  A is row-major n x n, overwritten.
+ * Returns 0, or -1 on an exactly zero pivot (x set to 0). */
+int or_lu_solve(int n, double *A, double *b, double *x);
+
+/* ---- Model_AB (Anderson / incompressible), src/Model_AB.cc ---------- */
+typedef struct or_model or_model;
+
+/* mde: 0 = RQM4, 1 = OS2.  contour: 0 = Simpson (Quad4_Closed when the
+ * slice count is even, src/Model_AB.cc:719-728), 1 = trapezoid.
+ * cell: crystal system code + a,b,c,alpha,beta,gamma (radians).
+ * Ms = slice counts [sA, sB] (src/Model_AB.cc:100-111 derives fA and ds).
+ * wA0/wB0: initial fields (File init), C-order. */
+or_model *or_model_create(int dim, int Lx, int Ly, int Lz,
+                          int cs, double a, double b, double c,
+                          double alpha, double beta, double gamma,
+                          int sA, int sB, double chiN,
+                          double lamA, double lamB, int n_anderson,
+                          int mde, int contour,
+                          const double *wA0, const double *wB0);
+void or_model_destroy(or_model *m);
+/* One Model_AB::update() (src/Model_AB.cc:136-146,195-199,231-253). */
+void or_model_update(or_model *m);
+/* Model_AB::H, Hw, Hs (src/Model_AB.cc:263-300) and residual_error
+ * (src/Model_AB.cc:307-315) evaluated after update(), as scft::relax does. */
+double or_model_H(const or_model *m);
+double or_model_Hw(const or_model *m);
+double or_model_Hs(const or_model *m);
+double or_model_residual(const or_model *m);
+double or_model_Q(const or_model *m);
+/* Copy out current fields / densities (M doubles each). which: 0 wA, 1 wB,
+ * 2 phiA, 3 phiB. */
+void or_model_get(const or_model *m, int which, double *out);
+/* scft::relax + save_data stopping rules (src/scft.cc:382-422,532-587):
+ * runs up to max_iter updates, records every record_interval, stops when
+ * residual < thresh after min_iter (with at least two records) or H is
+ * not finite.  Returns the number of iterations run; *H_out, *res_out get
+ * the last recorded values. */
+int or_model_relax(or_model *m, int max_iter, int min_iter,
+                   int record_interval, double thresh,
+                   double *H_out, double *res_out);
+
+/* Number of OpenMP threads the oracle uses (1 if built without OpenMP). */
+int or_num_threads(void);
+void or_set_num_threads(int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

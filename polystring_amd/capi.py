@@ -1,0 +1,237 @@
+"""ctypes binding of libpolystring_amd.so (include/polystring_amd.h).
+
+This is the thin Python side of the C ABI used by the tests, bench.py and the
+torch binding.  It loads the in-tree library and raises if it is missing: the
+HIP path has no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpolystring_amd.so")
+
+PO_OK = 0
+PO_ERR_INVALID, PO_ERR_DEVICE, PO_ERR_QUADRATURE, PO_ERR_STATE = 1, 2, 3, 4
+PO_FP64, PO_FP32_BULK = 0, 1
+PO_RQM4, PO_OS2 = 0, 1
+PO_QUAD_SIMPSON, PO_QUAD_QUAD4C, PO_QUAD_TRAP, PO_QUAD_AUTO = 0, 1, 2, 3
+FIELDS = {"wA": 0, "wB": 1, "phiA": 2, "phiB": 3}
+
+# CrystalSystemType order, include/common.h:149-163 of the reference
+CRYSTAL = {
+    "LAMELLAR": 0, "SQUARE": 1, "RECTANGULAR": 2, "HEXAGONAL": 3, "OBLIQUE": 4,
+    "CUBIC": 5, "TETRAGONAL": 6, "ORTHORHOMBIC": 7, "TRIGONAL": 8, "MONOCLINIC": 9,
+}
+
+# every symbol include/polystring_amd.h declares
+EXPORTS = [
+    "po_version", "po_last_error", "po_ctx_create", "po_ctx_destroy", "po_ctx_synchronize",
+    "po_shape_supported", "po_prop_create", "po_prop_destroy", "po_prop_len", "po_prop_set_head",
+    "po_prop_set_head_from_tail", "po_prop_solve", "po_prop_get_slice", "po_prop_Q", "po_density",
+    "po_model_create", "po_model_destroy", "po_model_update", "po_model_observe", "po_model_get",
+    "po_model_relax", "po_model_contour_steps",
+]
+
+
+class GridDesc(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int), ("Lx", ctypes.c_int), ("Ly", ctypes.c_int), ("Lz", ctypes.c_int),
+                ("crystal", ctypes.c_int), ("a", ctypes.c_double), ("b", ctypes.c_double),
+                ("c", ctypes.c_double), ("alpha", ctypes.c_double), ("beta", ctypes.c_double),
+                ("gamma", ctypes.c_double)]
+
+
+class ModelDesc(ctypes.Structure):
+    _fields_ = [("grid", GridDesc), ("sA", ctypes.c_int), ("sB", ctypes.c_int), ("chiN", ctypes.c_double),
+                ("lamA", ctypes.c_double), ("lamB", ctypes.c_double), ("n_anderson", ctypes.c_int),
+                ("algo", ctypes.c_int), ("contour", ctypes.c_int)]
+
+
+class POError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"polystring_amd error {code}: {msg}")
+        self.code = code
+
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_vp = ctypes.c_void_p
+_lib = None
+
+
+def load():
+    """Load the in-tree library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: run `python -m polystring_amd.build` first")
+    lib = ctypes.CDLL(LIB_PATH)
+    i, d = ctypes.c_int, ctypes.c_double
+    sig = {
+        "po_version": ([], i), "po_last_error": ([], ctypes.c_char_p),
+        "po_ctx_create": ([i, i, ctypes.POINTER(_vp)], i), "po_ctx_destroy": ([_vp], i),
+        "po_ctx_synchronize": ([_vp], i), "po_shape_supported": ([i, i, i], i),
+        "po_prop_create": ([_vp, ctypes.POINTER(GridDesc), i, i, d, ctypes.POINTER(_vp)], i),
+        "po_prop_destroy": ([_vp], i), "po_prop_len": ([_vp], i), "po_prop_set_head": ([_vp, _dp], i),
+        "po_prop_set_head_from_tail": ([_vp, _vp], i), "po_prop_solve": ([_vp, _dp], i),
+        "po_prop_get_slice": ([_vp, i, _dp], i), "po_prop_Q": ([_vp, i, _dp], i),
+        "po_density": ([_vp, _vp, i, d, _dp], i),
+        "po_model_create": ([_vp, ctypes.POINTER(ModelDesc), _dp, _dp, ctypes.POINTER(_vp)], i),
+        "po_model_destroy": ([_vp], i), "po_model_update": ([_vp, i], i), "po_model_observe": ([_vp, _dp], i),
+        "po_model_get": ([_vp, i, _dp], i),
+        "po_model_relax": ([_vp, i, i, i, d, ctypes.POINTER(i), _dp, _dp], i),
+        "po_model_contour_steps": ([_vp], ctypes.c_long),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = lib
+    return lib
+
+
+def _check(rc):
+    if rc != PO_OK:
+        raise POError(rc, load().po_last_error().decode())
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def grid_desc(shape, cell, dim=None):
+    """cell = (crystal, a, b, c, alpha, beta, gamma) with trailing entries optional."""
+    Lx, Ly, Lz = shape
+    if dim is None:
+        dim = 3 if Lz > 1 else (2 if Ly > 1 else 1)
+    vals = list(cell) + [0.0] * (7 - len(cell))
+    cs = CRYSTAL[vals[0].upper()] if isinstance(vals[0], str) else int(vals[0])
+    return GridDesc(dim, Lx, Ly, Lz, cs, *[float(v) for v in vals[1:7]])
+
+
+class Context:
+    """One GPU + one HIP stream + one precision ('fp64' or 'fp32')."""
+
+    def __init__(self, device=0, precision="fp64"):
+        self._lib = load()
+        h = _vp()
+        self.precision = precision
+        _check(self._lib.po_ctx_create(device, PO_FP64 if precision == "fp64" else PO_FP32_BULK, ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._lib.po_ctx_destroy(self.h)
+            self.h = None
+
+    def synchronize(self):
+        _check(self._lib.po_ctx_synchronize(self.h))
+
+    def __del__(self):
+        self.close()
+
+
+class Propagator:
+    """Propagator + its Updater (RQM4 or PseudoSpectral) on the GPU."""
+
+    def __init__(self, ctx, shape, cell, length, ds, algo="RQM4", dim=None):
+        self.ctx = ctx
+        self.shape = tuple(shape)
+        self.len = length
+        self.ds = ds
+        g = grid_desc(shape, cell, dim)
+        h = _vp()
+        _check(load().po_prop_create(ctx.h, ctypes.byref(g), PO_RQM4 if algo.upper() == "RQM4" else PO_OS2,
+                                     length, ds, ctypes.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            load().po_prop_destroy(self.h)
+            self.h = None
+
+    def set_head(self, q0):
+        q0 = np.ascontiguousarray(q0, dtype=np.float64)
+        assert q0.shape == self.shape
+        _check(load().po_prop_set_head(self.h, _p(q0)))
+
+    def set_head_from_tail(self, other):
+        _check(load().po_prop_set_head_from_tail(self.h, other.h))
+
+    def update(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        assert w.shape == self.shape
+        _check(load().po_prop_solve(self.h, _p(w)))
+
+    def __getitem__(self, s):
+        out = np.zeros(self.shape)
+        _check(load().po_prop_get_slice(self.h, int(s), _p(out)))
+        return out
+
+    def get_tail(self):
+        return self[self.len - 1]
+
+    def qs(self):
+        return np.stack([self[s] for s in range(self.len)])
+
+    def Q(self, s):
+        q = np.zeros(1)
+        _check(load().po_prop_Q(self.h, int(s), _p(q)))
+        return float(q[0])
+
+    def Qt(self):
+        return self.Q(self.len - 1)
+
+
+def density(q, qc, cc, quad="auto"):
+    kind = {"simpson": PO_QUAD_SIMPSON, "quad4": PO_QUAD_QUAD4C, "trap": PO_QUAD_TRAP, "auto": PO_QUAD_AUTO}[quad]
+    out = np.zeros(q.shape)
+    _check(load().po_density(q.h, qc.h, kind, float(cc), _p(out)))
+    return out
+
+
+class ModelAB:
+    """Model_AB (Anderson, incompressible) iterated on the GPU."""
+
+    def __init__(self, ctx, wA0, wB0, *, cell, Ms, chiN, lam=(0.1, 0.1), n_anderson=20, algo="RQM4",
+                 contour="simpson", dim=None):
+        wA0 = np.ascontiguousarray(wA0, dtype=np.float64)
+        wB0 = np.ascontiguousarray(wB0, dtype=np.float64)
+        self.shape = wA0.shape
+        self.ctx = ctx
+        d = ModelDesc(grid_desc(self.shape, cell, dim), int(Ms[0]), int(Ms[1]), float(chiN), float(lam[0]),
+                      float(lam[1]), int(n_anderson), PO_RQM4 if algo.upper() == "RQM4" else PO_OS2,
+                      PO_QUAD_TRAP if contour.lower() == "trap" else PO_QUAD_AUTO)
+        h = _vp()
+        _check(load().po_model_create(ctx.h, ctypes.byref(d), _p(wA0), _p(wB0), ctypes.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            load().po_model_destroy(self.h)
+            self.h = None
+
+    def update(self, n=1):
+        _check(load().po_model_update(self.h, int(n)))
+
+    def observe(self):
+        o = np.zeros(5)
+        _check(load().po_model_observe(self.h, _p(o)))
+        return dict(H=o[0], Hw=o[1], Hs=o[2], residual=o[3], Q=o[4])
+
+    def get(self, which):
+        out = np.zeros(self.shape)
+        _check(load().po_model_get(self.h, FIELDS[which], _p(out)))
+        return out
+
+    def relax(self, max_iter, min_iter=10, record_interval=1, thresh=1e-8):
+        it = ctypes.c_int(0)
+        H = np.zeros(1)
+        r = np.zeros(1)
+        _check(load().po_model_relax(self.h, max_iter, min_iter, record_interval, thresh, ctypes.byref(it),
+                                     _p(H), _p(r)))
+        return it.value, float(H[0]), float(r[0])
+
+    def contour_steps(self):
+        return load().po_model_contour_steps(self.h)

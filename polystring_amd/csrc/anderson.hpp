@@ -1,0 +1,41 @@
+// anderson.hpp -- device argument blocks of the Anderson-mixing kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace po {
+
+constexpr int kMaxHist = 32;  // largest n_Anderson_mixing supported on the device
+
+// Observables written by the device every iteration (indices into obs[]).
+enum ObsIndex { OBS_H = 0, OBS_HW, OBS_HS, OBS_RESIDUAL, OBS_Q, OBS_SINGULAR, OBS_COUNT };
+
+// Ring-buffer view of both species' Anderson histories (src/FieldAX.h:99-179):
+// wA/wB/dA/dB hold n+1 fields of M doubles; slot cur is d^k / w^k, hidx[j]
+// is the slot of d^{k-j} / w^{k-j} (FieldAX::find_index, src/FieldAX.cc:262-267).
+struct AndersonArgs {
+    double *wA, *wB, *dA, *dB;
+    long M;
+    int n, nn, nc, cur, next;
+    int hidx[kMaxHist + 1];
+    double chiN, lamA, lamB;
+};
+
+// Optional Boltzmann factor tables refreshed by the mixing kernel:
+// e = exp(c * w_new) with c = -ds/2 and -ds/4 per species.
+template <typename T> struct FactorOut {
+    T *eA2, *eA4, *eB2, *eB4;
+    double cA2, cA4, cB2, cB4;
+};
+
+hipError_t launch_anderson_d(const AndersonArgs &a, const double *phiA, const double *phiB, const double *wA,
+                             const double *wB, double *partials, hipStream_t st);
+hipError_t launch_anderson_solve(const AndersonArgs &a, const double *partials, double *BA, double *BB, double *C,
+                                 double *obs, hipStream_t st);
+template <typename T>
+hipError_t launch_anderson_mix(const AndersonArgs &a, const double *C, double *wA, double *wB, const double *phiA,
+                               const double *phiB, const FactorOut<T> &fo, double *partials, hipStream_t st);
+hipError_t launch_observables(const double *partials, long M, double *obs, hipStream_t st);
+hipError_t launch_finish_partials(const double *partials, int nblk, int nval, double div, double *out,
+                                  hipStream_t st);
+
+}  // namespace po

@@ -1,0 +1,884 @@
+// capi.hip -- host side of the C ABI (include/polystring_amd.h): device memory,
+// unit-cell tables, launch programs of the propagator chains, and the fused
+// Model_AB iteration.  Everything here is orchestration; the arithmetic is in
+// propagator.hip and kernels.hip.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/polystring_amd.h"
+#include "anderson.hpp"
+#include "engine.hpp"
+
+namespace po {
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define PO_HIP(x)                                                                                   \
+    do {                                                                                            \
+        hipError_t e_ = (x);                                                                        \
+        if (e_ != hipSuccess) return fail(PO_ERR_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T> struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    hipError_t alloc(size_t count) {
+        release();
+        n = count;
+        return hipMalloc((void **)&p, sizeof(T) * (count ? count : 1));
+    }
+};
+
+// ----------------------------------------------------------------- cell --
+// Shape matrices of UnitCell::calc_shape_matrix{1,2,3} (src/UnitCell.cc:142-296):
+// lengths l = (a, b, c) and reciprocal matrix g (row-major, 2 pi units).
+struct Cell {
+    int dim;
+    double l[3];
+    double g[9];
+    bool ortho;
+};
+
+static void cross3(const double *u, const double *v, double *o) {
+    o[0] = u[1] * v[2] - u[2] * v[1];
+    o[1] = u[2] * v[0] - u[0] * v[2];
+    o[2] = u[0] * v[1] - u[1] * v[0];
+}
+
+static int make_cell(const po_grid_desc &gd, Cell &c) {
+    const double pi = 3.14159265358979323846264338327950288;
+    const double pi2 = pi / 2.0, pi3 = 2.0 * pi / 3.0, twopi = 2.0 * pi;
+    double a = gd.a, b = gd.b, cc = gd.c, al = gd.alpha, be = gd.beta, ga = gd.gamma;
+    std::memset(&c, 0, sizeof(c));
+    c.dim = gd.dim;
+    if (gd.dim == 1) {
+        if (!(a > 0)) return fail(PO_ERR_INVALID, "cell length a must be positive");
+        c.l[0] = a;
+        c.g[0] = twopi / a;
+        c.g[4] = twopi;
+        c.g[8] = twopi;
+        c.ortho = true;
+        return PO_OK;
+    }
+    if (gd.dim == 2) {
+        if (!(a > 0)) return fail(PO_ERR_INVALID, "cell length a must be positive");
+        switch (gd.crystal) {
+        case 1: b = a; ga = pi2; break;                                       // SQUARE
+        case 2: if (!(b > 0)) return fail(PO_ERR_INVALID, "bad RECTANGULAR cell"); ga = pi2; break;
+        case 3: b = a; ga = pi3; break;                                       // HEXAGONAL
+        default: if (!(b > 0) || !(ga > 0)) return fail(PO_ERR_INVALID, "bad 2-D cell"); break;
+        }
+        const double A[3] = {a, 0, 0}, B[3] = {b * std::cos(ga), b * std::sin(ga), 0};
+        double acb[3], as[3], bs[3];
+        cross3(A, B, acb);
+        const double denom = acb[0] * acb[0] + acb[1] * acb[1] + acb[2] * acb[2];
+        cross3(B, acb, as);
+        cross3(acb, A, bs);
+        c.l[0] = a;
+        c.l[1] = b;
+        c.g[0] = twopi * as[0] / denom; c.g[1] = twopi * bs[0] / denom;
+        c.g[3] = twopi * as[1] / denom; c.g[4] = twopi * bs[1] / denom;
+        c.g[8] = twopi;
+        c.ortho = (ga == pi2);
+        return PO_OK;
+    }
+    if (gd.dim != 3) return fail(PO_ERR_INVALID, "dim must be 1, 2 or 3");
+    switch (gd.crystal) {
+    case 5: b = a; cc = a; al = be = ga = pi2; break;      // CUBIC
+    case 6: b = a; al = be = ga = pi2; break;              // TETRAGONAL
+    case 7: al = be = ga = pi2; break;                     // ORTHORHOMBIC
+    case 3: b = a; al = be = pi2; ga = pi3; break;         // HEXAGONAL
+    case 8: b = a; cc = a; al = ga; be = ga; break;        // TRIGONAL
+    case 9: al = pi2; ga = pi2; break;                     // MONOCLINIC
+    default: return fail(PO_ERR_INVALID, "unknown 3-D crystal system");
+    }
+    const double A[3] = {a, 0, 0}, B[3] = {b * std::cos(ga), b * std::sin(ga), 0};
+    const double cx = cc * std::cos(be), cy = cc * (std::cos(al) - std::cos(be) * std::cos(ga));
+    const double C[3] = {cx, cy, std::sqrt(cc * cc - cx * cx - cy * cy)};
+    double bcc[3], cca[3], acb[3];
+    cross3(B, C, bcc);
+    cross3(C, A, cca);
+    cross3(A, B, acb);
+    const double denom = A[0] * bcc[0] + A[1] * bcc[1] + A[2] * bcc[2];
+    for (int i = 0; i < 3; ++i) {
+        c.g[3 * i + 0] = twopi * bcc[i] / denom;
+        c.g[3 * i + 1] = twopi * cca[i] / denom;
+        c.g[3 * i + 2] = twopi * acb[i] / denom;
+    }
+    c.l[0] = a;
+    c.l[1] = b;
+    c.l[2] = cc;
+    c.ortho = (al == pi2 && be == pi2 && ga == pi2);
+    return PO_OK;
+}
+
+// minimum-image index r = i for i < L/2+1, else L - i (src/UnitCell.cc:399-412)
+static inline int rmin(int i, int L) { return i < L / 2 + 1 ? i : L - i; }
+
+// ----------------------------------------------------------- k factors --
+// K(k) = exp(h * (-|k|^2)).  Orthogonal tables (RQM4 always, and OS2 on an
+// orthogonal cell) are separable: K = fx[kx] fy[ky] fz[kz] with
+// fx[i] = exp(h * (-4 pi^2 / lx^2) * r_i^2) (src/UnitCell.cc:368-418).  OS2 on
+// a general cell uses the full half-spectrum table of
+// exp(-h * min over 27 images |g (n + shift)|^2) (src/UnitCell.cc:321-359).
+template <typename T> struct KTable {
+    DBuf<T> fx, fy, fz, tab;
+    XDesc<T> desc(cx<T> *S) const { return XDesc<T>{S, fx.p, fy.p, fz.p, tab.p}; }
+};
+
+template <typename T>
+static int build_ktable(KTable<T> &kt, const Shape &sh, const Cell &cell, bool general, double h, hipStream_t st) {
+    const double pi = 3.14159265358979323846264338327950288;
+    if (!general) {
+        double cc[3] = {0, 0, 0};
+        for (int d = 0; d < cell.dim; ++d) cc[d] = -4.0 * pi * pi / (cell.l[d] * cell.l[d]);
+        std::vector<T> fx(sh.Lx), fy(sh.Ly), fz(sh.Hz);
+        for (int i = 0; i < sh.Lx; ++i) { const int r = rmin(i, sh.Lx); fx[i] = (T)std::exp(h * (cc[0] * r * r)); }
+        for (int j = 0; j < sh.Ly; ++j) { const int r = rmin(j, sh.Ly); fy[j] = (T)std::exp(h * (cc[1] * r * r)); }
+        for (int k = 0; k < sh.Hz; ++k) { const int r = rmin(k, sh.Lz); fz[k] = (T)std::exp(h * (cc[2] * r * r)); }
+        PO_HIP(kt.fx.alloc(sh.Lx));
+        PO_HIP(kt.fy.alloc(sh.Ly));
+        PO_HIP(kt.fz.alloc(sh.Hz));
+        PO_HIP(hipMemcpyAsync(kt.fx.p, fx.data(), sizeof(T) * sh.Lx, hipMemcpyHostToDevice, st));
+        PO_HIP(hipMemcpyAsync(kt.fy.p, fy.data(), sizeof(T) * sh.Ly, hipMemcpyHostToDevice, st));
+        PO_HIP(hipMemcpyAsync(kt.fz.p, fz.data(), sizeof(T) * sh.Hz, hipMemcpyHostToDevice, st));
+        PO_HIP(hipStreamSynchronize(st));
+        return PO_OK;
+    }
+    const int N[3] = {sh.Lx, sh.Ly, sh.Lz};
+    std::vector<T> tab((size_t)sh.Mh);
+    for (int i = 0; i < sh.Lx; ++i)
+        for (int j = 0; j < sh.Ly; ++j)
+            for (int k = 0; k < sh.Hz; ++k) {
+                double q2min = 1e99;
+                for (int a = 1; a > -2; --a)
+                    for (int b = 1; b > -2; --b)
+                        for (int c = 1; c > -2; --c) {
+                            const double bq[3] = {(double)(a * N[0]) + i, (double)(b * N[1]) + j,
+                                                  (double)(c * N[2]) + k};
+                            double q2 = 0.0;
+                            for (int r = 0; r < 3; ++r) {
+                                const double qt = cell.g[3 * r] * bq[0] + cell.g[3 * r + 1] * bq[1] +
+                                                  cell.g[3 * r + 2] * bq[2];
+                                q2 += qt * qt;
+                            }
+                            if (q2 < q2min) q2min = q2;
+                        }
+                tab[((size_t)i * sh.Ly + j) * sh.Hz + k] = (T)std::exp(-h * q2min);
+            }
+    PO_HIP(kt.tab.alloc(sh.Mh));
+    PO_HIP(hipMemcpyAsync(kt.tab.p, tab.data(), sizeof(T) * sh.Mh, hipMemcpyHostToDevice, st));
+    PO_HIP(hipStreamSynchronize(st));
+    return PO_OK;
+}
+
+}  // namespace po
+
+// ----------------------------------------------------------------- context --
+struct po_ctx {
+    int device = 0;
+    int precision = PO_FP64;
+    hipStream_t stream = nullptr;
+    std::map<int, std::unique_ptr<po::DBuf<double2>>> roots64;
+    std::map<int, std::unique_ptr<po::DBuf<float2>>> roots32;
+};
+
+namespace po {
+
+template <typename T> struct RootsMap;
+template <> struct RootsMap<double> {
+    static std::map<int, std::unique_ptr<DBuf<double2>>> &get(po_ctx *c) { return c->roots64; }
+};
+template <> struct RootsMap<float> {
+    static std::map<int, std::unique_ptr<DBuf<float2>>> &get(po_ctx *c) { return c->roots32; }
+};
+
+// N-th roots of unity exp(-2 pi i m / N), computed in long double, cached per context
+template <typename T> static int roots_for(po_ctx *ctx, int N, const cx<T> **out) {
+    auto &m = RootsMap<T>::get(ctx);
+    const int key = N < 1 ? 1 : N;
+    auto it = m.find(key);
+    if (it == m.end()) {
+        std::vector<cx<T>> h(key);
+        for (int i = 0; i < key; ++i) {
+            const long double ang = -2.0L * 3.141592653589793238462643383279502884L * i / key;
+            h[i].x = (T)cosl(ang);
+            h[i].y = (T)sinl(ang);
+        }
+        auto buf = std::make_unique<DBuf<cx<T>>>();
+        PO_HIP(buf->alloc(key));
+        PO_HIP(hipMemcpy(buf->p, h.data(), sizeof(cx<T>) * key, hipMemcpyHostToDevice));
+        it = m.emplace(key, std::move(buf)).first;
+    }
+    *out = it->second->p;
+    return PO_OK;
+}
+
+template <typename T> static int make_roots(po_ctx *ctx, const Shape &sh, Roots<T> &rt) {
+    int rc;
+    if ((rc = roots_for<T>(ctx, sh.Lx, &rt.x))) return rc;
+    if ((rc = roots_for<T>(ctx, sh.Ly, &rt.y))) return rc;
+    if ((rc = roots_for<T>(ctx, sh.Lz / 2, &rt.zh))) return rc;
+    if ((rc = roots_for<T>(ctx, sh.Lz, &rt.z))) return rc;
+    return PO_OK;
+}
+
+static int make_shape(const po_grid_desc &g, Shape &sh) {
+    if (g.Lx < 1 || g.Ly < 1 || g.Lz < 1) return fail(PO_ERR_INVALID, "grid sizes must be positive");
+    if (!plane_supported(g.Ly, g.Lz) || !xpass_supported(g.Lx))
+        return fail(PO_ERR_INVALID, "grid " + std::to_string(g.Lx) + "x" + std::to_string(g.Ly) + "x" +
+                                        std::to_string(g.Lz) + " has no compiled kernels");
+    sh.Lx = g.Lx;
+    sh.Ly = g.Ly;
+    sh.Lz = g.Lz;
+    sh.Hz = g.Lz == 1 ? 1 : g.Lz / 2 + 1;
+    sh.M = (long)g.Lx * g.Ly * g.Lz;
+    sh.Mh = (long)g.Lx * g.Ly * sh.Hz;
+    return PO_OK;
+}
+
+// ------------------------------------------------------- chain programs --
+// A lane is a sequence of propagator segments solved back to back, the head
+// of each segment being the tail of the previous one (qB after qA, qAc after
+// qBc: src/Model_AB.cc:139-145).  Lanes of one program advance in lockstep,
+// one launch per pass for all lanes.
+template <typename T> struct Segment {
+    T *q;
+    int len;
+    const T *e2, *e4;            // exp(-w ds/2), exp(-w ds/4)
+    const KTable<T> *kfull, *khalf;  // K for h = ds and h = ds/2
+};
+
+template <typename T> struct Lane {
+    std::vector<Segment<T>> segs;
+    cx<T> *Sf, *Sh;
+};
+
+template <typename T> struct Program {
+    struct Launch {
+        bool plane;
+        int off, n;
+    };
+    DBuf<PlaneDesc<T>> pd;
+    DBuf<XDesc<T>> xd;
+    std::vector<Launch> seq;
+
+    int build(int algo, const std::vector<Lane<T>> &lanes, hipStream_t st) {
+        const int L = (int)lanes.size();
+        int nsteps = -1;
+        for (const auto &ln : lanes) {
+            int s = 0;
+            for (const auto &sg : ln.segs) {
+                if (sg.len < 2) return fail(PO_ERR_INVALID, "propagator length must be >= 2");
+                s += sg.len - 1;
+            }
+            if (nsteps >= 0 && s != nsteps) return fail(PO_ERR_INVALID, "lockstep lanes need equal step counts");
+            nsteps = s;
+        }
+        std::vector<PlaneDesc<T>> P;
+        std::vector<XDesc<T>> X;
+        seq.clear();
+        auto at = [&](int lane, int t, int &si, int &s) {
+            si = 0;
+            s = t;
+            while (s >= lanes[lane].segs[si].len - 1) { s -= lanes[lane].segs[si].len - 1; ++si; }
+        };
+        auto push_p = [&](std::vector<PlaneDesc<T>> &v) { seq.push_back({true, (int)P.size(), (int)v.size()}); P.insert(P.end(), v.begin(), v.end()); };
+        auto push_x = [&](std::vector<XDesc<T>> &v) { seq.push_back({false, (int)X.size(), (int)v.size()}); X.insert(X.end(), v.begin(), v.end()); };
+        const bool rqm4 = algo == PO_RQM4;
+        {
+            std::vector<PlaneDesc<T>> v;
+            for (int l = 0; l < L; ++l) {
+                const auto &sg = lanes[l].segs[0];
+                PlaneDesc<T> d{};
+                d.qin = sg.q;
+                d.out0 = lanes[l].Sf;
+                d.pre0 = sg.e2;
+                if (rqm4) { d.out1 = lanes[l].Sh; d.pre1 = sg.e4; }
+                v.push_back(d);
+            }
+            push_p(v);
+        }
+        for (int t = 0; t < nsteps; ++t) {
+            std::vector<XDesc<T>> x2, x1;
+            std::vector<PlaneDesc<T>> mid, fin;
+            for (int l = 0; l < L; ++l) {
+                int si, s;
+                at(l, t, si, s);
+                const auto &sg = lanes[l].segs[si];
+                PlaneDesc<T> f{};
+                f.in0 = lanes[l].Sf;
+                f.post0 = sg.e2;
+                if (rqm4) {
+                    x2.push_back(sg.kfull->desc(lanes[l].Sf));
+                    x2.push_back(sg.khalf->desc(lanes[l].Sh));
+                    PlaneDesc<T> m{};
+                    m.in0 = lanes[l].Sh;
+                    m.post0 = sg.e4;
+                    m.out0 = lanes[l].Sh;
+                    m.pre0 = sg.e4;
+                    mid.push_back(m);
+                    x1.push_back(sg.khalf->desc(lanes[l].Sh));
+                    f.in1 = lanes[l].Sh;
+                    f.post1 = sg.e4;
+                } else {
+                    x2.push_back(sg.kfull->desc(lanes[l].Sf));
+                }
+                f.qout = sg.q + (size_t)(s + 1) * slice_elems;
+                if (s + 1 == sg.len - 1 && si + 1 < (int)lanes[l].segs.size()) f.qout2 = lanes[l].segs[si + 1].q;
+                if (t + 1 < nsteps) {
+                    int sn, ssn;
+                    at(l, t + 1, sn, ssn);
+                    const auto &nx = lanes[l].segs[sn];
+                    f.out0 = lanes[l].Sf;
+                    f.pre0 = nx.e2;
+                    if (rqm4) { f.out1 = lanes[l].Sh; f.pre1 = nx.e4; }
+                }
+                fin.push_back(f);
+            }
+            push_x(x2);
+            if (rqm4) {
+                push_p(mid);
+                push_x(x1);
+            }
+            push_p(fin);
+        }
+        PO_HIP(pd.alloc(P.size()));
+        PO_HIP(xd.alloc(X.size() ? X.size() : 1));
+        PO_HIP(hipMemcpyAsync(pd.p, P.data(), sizeof(PlaneDesc<T>) * P.size(), hipMemcpyHostToDevice, st));
+        if (!X.empty()) PO_HIP(hipMemcpyAsync(xd.p, X.data(), sizeof(XDesc<T>) * X.size(), hipMemcpyHostToDevice, st));
+        PO_HIP(hipStreamSynchronize(st));
+        return PO_OK;
+    }
+
+    int run(const Shape &sh, const Roots<T> &rt, hipStream_t st) const {
+        for (const auto &l : seq) {
+            hipError_t e = l.plane ? launch_plane<T>(sh, rt, pd.p + l.off, l.n, st)
+                                   : launch_xpass<T>(sh, rt, xd.p + l.off, l.n, st);
+            if (e != hipSuccess) return fail(PO_ERR_DEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
+        }
+        return PO_OK;
+    }
+
+    long slice_elems = 0;  // M, set before build()
+};
+
+}  // namespace po
+
+// ------------------------------------------------------------- propagator --
+struct po_prop {
+    po_ctx *ctx = nullptr;
+    int algo = PO_RQM4;
+    int len = 0;
+    double ds = 0.0;
+    po::Shape sh{};
+    po::Cell cell{};
+    bool solved = false;
+    virtual ~po_prop() = default;
+    virtual int set_head(const double *h) = 0;
+    virtual int solve(const double *w) = 0;
+    virtual int get_slice(int s, double *out) const = 0;
+    virtual const void *slice_ptr(int s) const = 0;
+    virtual int Q(int s, double *q) const = 0;
+};
+
+namespace po {
+
+template <typename T> struct Prop : po_prop {
+    DBuf<T> q, e2, e4;
+    DBuf<double> w, part, scal;
+    DBuf<cx<T>> Sf, Sh;
+    KTable<T> kfull, khalf;
+    Roots<T> rt{};
+    Program<T> prog;
+
+    int init(po_ctx *c, const po_grid_desc &g, int a, int l, double d) {
+        ctx = c;
+        algo = a;
+        len = l;
+        ds = d;
+        int rc;
+        if ((rc = make_shape(g, sh))) return rc;
+        if ((rc = make_cell(g, cell))) return rc;
+        if ((rc = make_roots<T>(c, sh, rt))) return rc;
+        hipStream_t st = c->stream;
+        PO_HIP(q.alloc((size_t)len * sh.M));
+        PO_HIP(e2.alloc(sh.M));
+        PO_HIP(e4.alloc(sh.M));
+        PO_HIP(w.alloc(sh.M));
+        PO_HIP(part.alloc(kRedBlocks));
+        PO_HIP(scal.alloc(1));
+        PO_HIP(Sf.alloc(sh.Mh));
+        PO_HIP(Sh.alloc(sh.Mh));
+        const bool general = (algo == PO_OS2) && !cell.ortho;
+        if ((rc = build_ktable<T>(kfull, sh, cell, general, ds, st))) return rc;
+        if (algo == PO_RQM4 && (rc = build_ktable<T>(khalf, sh, cell, false, 0.5 * ds, st))) return rc;
+        std::vector<Lane<T>> lanes(1);
+        lanes[0].segs.push_back(Segment<T>{q.p, len, e2.p, e4.p, &kfull, &khalf});
+        lanes[0].Sf = Sf.p;
+        lanes[0].Sh = Sh.p;
+        prog.slice_elems = sh.M;
+        return prog.build(algo, lanes, st);
+    }
+
+    int set_head(const double *h) override {
+        std::vector<T> t(sh.M);
+        for (long i = 0; i < sh.M; ++i) t[i] = (T)h[i];
+        PO_HIP(hipMemcpyAsync(q.p, t.data(), sizeof(T) * sh.M, hipMemcpyHostToDevice, ctx->stream));
+        PO_HIP(hipStreamSynchronize(ctx->stream));
+        return PO_OK;
+    }
+
+    int solve(const double *hw) override {
+        hipStream_t st = ctx->stream;
+        PO_HIP(hipMemcpyAsync(w.p, hw, sizeof(double) * sh.M, hipMemcpyHostToDevice, st));
+        PO_HIP(launch_exp_factor<T>(w.p, e2.p, -0.5 * ds, sh.M, st));
+        if (algo == PO_RQM4) PO_HIP(launch_exp_factor<T>(w.p, e4.p, -0.25 * ds, sh.M, st));
+        int rc = prog.run(sh, rt, st);
+        if (rc) return rc;
+        PO_HIP(hipStreamSynchronize(st));
+        solved = true;
+        return PO_OK;
+    }
+
+    int get_slice(int s, double *out) const override {
+        if (s < 0 || s >= len) return fail(PO_ERR_INVALID, "slice index out of range");
+        std::vector<T> t(sh.M);
+        PO_HIP(hipMemcpy(t.data(), q.p + (size_t)s * sh.M, sizeof(T) * sh.M, hipMemcpyDeviceToHost));
+        for (long i = 0; i < sh.M; ++i) out[i] = (double)t[i];
+        return PO_OK;
+    }
+
+    const void *slice_ptr(int s) const override { return q.p + (size_t)s * sh.M; }
+
+    int Q(int s, double *out) const override {
+        if (s < 0 || s >= len) return fail(PO_ERR_INVALID, "slice index out of range");
+        hipStream_t st = ctx->stream;
+        PO_HIP(launch_mean_partials<T>(q.p + (size_t)s * sh.M, sh.M, part.p, st));
+        PO_HIP(launch_finish_partials(part.p, kRedBlocks, 1, (double)sh.M, scal.p, st));
+        PO_HIP(hipMemcpyAsync(out, scal.p, sizeof(double), hipMemcpyDeviceToHost, st));
+        PO_HIP(hipStreamSynchronize(st));
+        return PO_OK;
+    }
+};
+
+}  // namespace po
+
+// ------------------------------------------------------------------ model --
+struct po_model {
+    po_ctx *ctx = nullptr;
+    po_model_desc desc{};
+    long steps = 0;
+    virtual ~po_model() = default;
+    virtual int update(int n) = 0;
+    virtual int observe(double *out) = 0;
+    virtual int get(int which, double *out) = 0;
+};
+
+namespace po {
+
+template <typename T> struct Model : po_model {
+    Shape sh{};
+    Cell cell{};
+    Roots<T> rt{};
+    int sA = 0, sB = 0, n = 0, cur = 0;
+    long k = 0;
+    double fA = 0, fB = 0, dsA = 0, dsB = 0;
+    int quadA = 0, quadB = 0;
+    DBuf<double> wA, wB, wAh, wBh, dAh, dBh, BA, BB, C, phiA, phiB, part, obs;
+    DBuf<T> eA2, eA4, eB2, eB4, qA, qB, qAc, qBc;
+    DBuf<cx<T>> S[4];
+    KTable<T> kA, kAh, kB, kBh;
+    Program<T> prog;
+
+    int init(po_ctx *c, const po_model_desc &d, const double *wA0, const double *wB0) {
+        ctx = c;
+        desc = d;
+        int rc;
+        if ((rc = make_shape(d.grid, sh))) return rc;
+        if ((rc = make_cell(d.grid, cell))) return rc;
+        if ((rc = make_roots<T>(c, sh, rt))) return rc;
+        if (d.sA < 2 || d.sB < 2) return fail(PO_ERR_INVALID, "Ms entries must be >= 2");
+        if (d.n_anderson < 0 || d.n_anderson > kMaxHist)
+            return fail(PO_ERR_INVALID, "n_Anderson_mixing must be in [0, " + std::to_string(kMaxHist) + "]");
+        if (d.algo != PO_RQM4 && d.algo != PO_OS2) return fail(PO_ERR_INVALID, "algo must be RQM4 or OS2");
+        sA = d.sA;
+        sB = d.sB;
+        n = d.n_anderson;
+        // src/Model_AB.cc:100-111
+        const int LA = sA - 1, LB = sB - 1;
+        fA = 1.0 * LA / (LA + LB);
+        fB = 1.0 * LB / (LA + LB);
+        dsA = fA / LA;
+        dsB = fB / LB;
+        // src/Model_AB.cc:713-735
+        if (d.contour == PO_QUAD_TRAP) {
+            quadA = quadB = QUAD_TRAP;
+        } else {
+            quadA = sA % 2 != 0 ? QUAD_SIMPSON : QUAD_QUAD4C;
+            quadB = sB % 2 != 0 ? QUAD_SIMPSON : QUAD_QUAD4C;
+            if ((quadA == QUAD_SIMPSON && LA < 4) || (quadA == QUAD_QUAD4C && LA < 7) ||
+                (quadB == QUAD_SIMPSON && LB < 4) || (quadB == QUAD_QUAD4C && LB < 7))
+                return fail(PO_ERR_QUADRATURE, "too few contour slices for the quadrature");
+        }
+        steps = (long)LA + LB;
+        steps *= 2;
+        const long M = sh.M;
+        hipStream_t st = c->stream;
+        PO_HIP(wA.alloc(M));
+        PO_HIP(wB.alloc(M));
+        PO_HIP(wAh.alloc((size_t)(n + 1) * M));
+        PO_HIP(wBh.alloc((size_t)(n + 1) * M));
+        PO_HIP(dAh.alloc((size_t)(n + 1) * M));
+        PO_HIP(dBh.alloc((size_t)(n + 1) * M));
+        PO_HIP(BA.alloc((size_t)(n + 1) * (n + 1)));
+        PO_HIP(BB.alloc((size_t)(n + 1) * (n + 1)));
+        PO_HIP(C.alloc(n + 1));
+        PO_HIP(phiA.alloc(M));
+        PO_HIP(phiB.alloc(M));
+        PO_HIP(part.alloc((size_t)kRedBlocks * (2 * (kMaxHist + 1) + 2)));
+        PO_HIP(obs.alloc(OBS_COUNT));
+        PO_HIP(eA2.alloc(M));
+        PO_HIP(eA4.alloc(M));
+        PO_HIP(eB2.alloc(M));
+        PO_HIP(eB4.alloc(M));
+        PO_HIP(qA.alloc((size_t)sA * M));
+        PO_HIP(qAc.alloc((size_t)sA * M));
+        PO_HIP(qB.alloc((size_t)sB * M));
+        PO_HIP(qBc.alloc((size_t)sB * M));
+        for (auto &s : S) PO_HIP(s.alloc(sh.Mh));
+        PO_HIP(hipMemsetAsync(BA.p, 0, sizeof(double) * BA.n, st));
+        PO_HIP(hipMemsetAsync(BB.p, 0, sizeof(double) * BB.n, st));
+        PO_HIP(hipMemsetAsync(dAh.p, 0, sizeof(double) * dAh.n, st));
+        PO_HIP(hipMemsetAsync(dBh.p, 0, sizeof(double) * dBh.n, st));
+        PO_HIP(hipMemsetAsync(obs.p, 0, sizeof(double) * OBS_COUNT, st));
+        // FieldAX(..., file, lam, n): w(n) = w0, cur_pos = n (src/FieldAX.cc:34-39,274-280)
+        cur = n;
+        k = 0;
+        PO_HIP(hipMemcpyAsync(wA.p, wA0, sizeof(double) * M, hipMemcpyHostToDevice, st));
+        PO_HIP(hipMemcpyAsync(wB.p, wB0, sizeof(double) * M, hipMemcpyHostToDevice, st));
+        PO_HIP(hipMemcpyAsync(wAh.p + (size_t)n * M, wA0, sizeof(double) * M, hipMemcpyHostToDevice, st));
+        PO_HIP(hipMemcpyAsync(wBh.p + (size_t)n * M, wB0, sizeof(double) * M, hipMemcpyHostToDevice, st));
+        // heads qA[0] = qBc[0] = 1 (src/Model_AB.cc:779-782)
+        PO_HIP(launch_fill<T>(qA.p, (T)1, M, st));
+        PO_HIP(launch_fill<T>(qBc.p, (T)1, M, st));
+        PO_HIP(launch_exp_factor<T>(wA.p, eA2.p, -0.5 * dsA, M, st));
+        PO_HIP(launch_exp_factor<T>(wA.p, eA4.p, -0.25 * dsA, M, st));
+        PO_HIP(launch_exp_factor<T>(wB.p, eB2.p, -0.5 * dsB, M, st));
+        PO_HIP(launch_exp_factor<T>(wB.p, eB4.p, -0.25 * dsB, M, st));
+        const bool general = (d.algo == PO_OS2) && !cell.ortho;
+        if ((rc = build_ktable<T>(kA, sh, cell, general, dsA, st))) return rc;
+        if ((rc = build_ktable<T>(kB, sh, cell, general, dsB, st))) return rc;
+        if (d.algo == PO_RQM4) {
+            if ((rc = build_ktable<T>(kAh, sh, cell, false, 0.5 * dsA, st))) return rc;
+            if ((rc = build_ktable<T>(kBh, sh, cell, false, 0.5 * dsB, st))) return rc;
+        }
+        // lane 0: qA -> qB ; lane 1: qBc -> qAc  (src/Model_AB.cc:139-145)
+        std::vector<Lane<T>> lanes(2);
+        lanes[0].segs = {Segment<T>{qA.p, sA, eA2.p, eA4.p, &kA, &kAh}, Segment<T>{qB.p, sB, eB2.p, eB4.p, &kB, &kBh}};
+        lanes[0].Sf = S[0].p;
+        lanes[0].Sh = S[1].p;
+        lanes[1].segs = {Segment<T>{qBc.p, sB, eB2.p, eB4.p, &kB, &kBh}, Segment<T>{qAc.p, sA, eA2.p, eA4.p, &kA, &kAh}};
+        lanes[1].Sf = S[2].p;
+        lanes[1].Sh = S[3].p;
+        prog.slice_elems = M;
+        return prog.build(d.algo, lanes, st);
+    }
+
+    int find_index(int j) const {  // src/FieldAX.cc:262-267
+        int i = j + cur;
+        if (i > n) i -= n + 1;
+        return i;
+    }
+
+    int update(int iters) override {
+        hipStream_t st = ctx->stream;
+        const long M = sh.M;
+        for (int it = 0; it < iters; ++it) {
+            int rc = prog.run(sh, rt, st);
+            if (rc) return rc;
+            // Q = qB->Qt() (src/Model_AB.cc:195)
+            PO_HIP(launch_mean_partials<T>(qB.p + (size_t)(sB - 1) * M, M, part.p, st));
+            PO_HIP(launch_finish_partials(part.p, kRedBlocks, 1, (double)M, obs.p + OBS_Q, st));
+            PO_HIP(launch_density<T>(qA.p, qAc.p, sA, M, quadA, obs.p + OBS_Q, 0.0, dsA, phiA.p, st));
+            PO_HIP(launch_density<T>(qB.p, qBc.p, sB, M, quadB, obs.p + OBS_Q, 0.0, dsB, phiB.p, st));
+            AndersonArgs a{};
+            a.wA = wAh.p;
+            a.wB = wBh.p;
+            a.dA = dAh.p;
+            a.dB = dBh.p;
+            a.M = M;
+            a.n = n;
+            a.nn = k < n ? (int)k : n;
+            a.nc = a.nn;
+            a.cur = cur;
+            a.next = cur > 0 ? cur - 1 : n;
+            for (int j = 0; j <= a.nn; ++j) a.hidx[j] = find_index(j);
+            a.chiN = desc.chiN;
+            a.lamA = 1.0 - std::pow(desc.lamA, (double)(k + 1));
+            a.lamB = 1.0 - std::pow(desc.lamB, (double)(k + 1));
+            PO_HIP(launch_anderson_d(a, phiA.p, phiB.p, wA.p, wB.p, part.p, st));
+            PO_HIP(launch_anderson_solve(a, part.p, BA.p, BB.p, C.p, obs.p, st));
+            FactorOut<T> fo{eA2.p, eA4.p, eB2.p, eB4.p, -0.5 * dsA, -0.25 * dsA, -0.5 * dsB, -0.25 * dsB};
+            PO_HIP(launch_anderson_mix<T>(a, C.p, wA.p, wB.p, phiA.p, phiB.p, fo, part.p, st));
+            PO_HIP(launch_observables(part.p, M, obs.p, st));
+            cur = a.next;
+            ++k;
+        }
+        return PO_OK;
+    }
+
+    int observe(double *out) override {
+        double o[OBS_COUNT];
+        PO_HIP(hipMemcpyAsync(o, obs.p, sizeof(o), hipMemcpyDeviceToHost, ctx->stream));
+        PO_HIP(hipStreamSynchronize(ctx->stream));
+        out[0] = o[OBS_H];
+        out[1] = o[OBS_HW];
+        out[2] = o[OBS_HS];
+        out[3] = o[OBS_RESIDUAL];
+        out[4] = o[OBS_Q];
+        return PO_OK;
+    }
+
+    int get(int which, double *out) override {
+        const double *src = which == PO_FIELD_WA ? wA.p : which == PO_FIELD_WB ? wB.p
+                          : which == PO_FIELD_PHIA ? phiA.p : which == PO_FIELD_PHIB ? phiB.p : nullptr;
+        if (!src) return fail(PO_ERR_INVALID, "unknown field");
+        PO_HIP(hipMemcpyAsync(out, src, sizeof(double) * sh.M, hipMemcpyDeviceToHost, ctx->stream));
+        PO_HIP(hipStreamSynchronize(ctx->stream));
+        return PO_OK;
+    }
+};
+
+}  // namespace po
+
+// ------------------------------------------------------------------- C ABI --
+using po::fail;
+
+extern "C" {
+
+int po_version(void) { return 100; }
+
+const char *po_last_error(void) { return po::g_err.c_str(); }
+
+int po_ctx_create(int device, int precision, po_ctx **out) {
+    if (!out) return fail(PO_ERR_INVALID, "null output pointer");
+    if (precision != PO_FP64 && precision != PO_FP32_BULK) return fail(PO_ERR_INVALID, "unknown precision");
+    int ndev = 0;
+    PO_HIP(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(PO_ERR_INVALID, "device index out of range");
+    PO_HIP(hipSetDevice(device));
+    auto *c = new po_ctx();
+    c->device = device;
+    c->precision = precision;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(PO_ERR_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    *out = c;
+    return PO_OK;
+}
+
+int po_ctx_destroy(po_ctx *ctx) {
+    if (!ctx) return PO_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx->roots64.clear();
+    ctx->roots32.clear();
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return PO_OK;
+}
+
+int po_ctx_synchronize(po_ctx *ctx) {
+    if (!ctx) return fail(PO_ERR_INVALID, "null context");
+    PO_HIP(hipStreamSynchronize(ctx->stream));
+    return PO_OK;
+}
+
+int po_shape_supported(int Lx, int Ly, int Lz) { return po::plane_supported(Ly, Lz) && po::xpass_supported(Lx); }
+
+int po_prop_create(po_ctx *ctx, const po_grid_desc *grid, int algo, int len, double ds, po_prop **out) {
+    if (!ctx || !grid || !out) return fail(PO_ERR_INVALID, "null argument");
+    if (algo != PO_RQM4 && algo != PO_OS2) return fail(PO_ERR_INVALID, "algo must be PO_RQM4 or PO_OS2");
+    if (len < 2) return fail(PO_ERR_INVALID, "len must be >= 2");
+    PO_HIP(hipSetDevice(ctx->device));
+    po_prop *p;
+    int rc;
+    if (ctx->precision == PO_FP64) {
+        auto *q = new po::Prop<double>();
+        rc = q->init(ctx, *grid, algo, len, ds);
+        p = q;
+    } else {
+        auto *q = new po::Prop<float>();
+        rc = q->init(ctx, *grid, algo, len, ds);
+        p = q;
+    }
+    if (rc) {
+        delete p;
+        return rc;
+    }
+    *out = p;
+    return PO_OK;
+}
+
+int po_prop_destroy(po_prop *p) {
+    delete p;
+    return PO_OK;
+}
+
+int po_prop_len(const po_prop *p) { return p ? p->len : -1; }
+
+int po_prop_set_head(po_prop *p, const double *h) {
+    if (!p || !h) return fail(PO_ERR_INVALID, "null argument");
+    return p->set_head(h);
+}
+
+int po_prop_set_head_from_tail(po_prop *dst, const po_prop *src) {
+    if (!dst || !src) return fail(PO_ERR_INVALID, "null argument");
+    if (dst->ctx != src->ctx || dst->sh.M != src->sh.M) return fail(PO_ERR_INVALID, "propagators differ in context or grid");
+    const size_t bytes = (size_t)src->sh.M * (dst->ctx->precision == PO_FP64 ? sizeof(double) : sizeof(float));
+    PO_HIP(hipMemcpyAsync(const_cast<void *>(dst->slice_ptr(0)), src->slice_ptr(src->len - 1), bytes,
+                          hipMemcpyDeviceToDevice, dst->ctx->stream));
+    PO_HIP(hipStreamSynchronize(dst->ctx->stream));
+    return PO_OK;
+}
+
+int po_prop_solve(po_prop *p, const double *w) {
+    if (!p || !w) return fail(PO_ERR_INVALID, "null argument");
+    return p->solve(w);
+}
+
+int po_prop_get_slice(const po_prop *p, int s, double *out) {
+    if (!p || !out) return fail(PO_ERR_INVALID, "null argument");
+    return p->get_slice(s, out);
+}
+
+int po_prop_Q(const po_prop *p, int s, double *Q) {
+    if (!p || !Q) return fail(PO_ERR_INVALID, "null argument");
+    return p->Q(s, Q);
+}
+
+int po_density(const po_prop *q, const po_prop *qc, int quad, double cc, double *host_phi) {
+    if (!q || !qc || !host_phi) return fail(PO_ERR_INVALID, "null argument");
+    if (q->ctx != qc->ctx || q->sh.M != qc->sh.M || q->len != qc->len)
+        return fail(PO_ERR_INVALID, "q and qc differ in context, grid or length");
+    const int len = q->len, nsub = len - 1;
+    if (quad == PO_QUAD_AUTO) quad = len % 2 != 0 ? PO_QUAD_SIMPSON : PO_QUAD_QUAD4C;
+    if (quad == PO_QUAD_SIMPSON && (nsub % 2 != 0 || nsub < 4))
+        return fail(PO_ERR_QUADRATURE, "Simpson requires an even number (>= 4) of subintervals");
+    if (quad == PO_QUAD_QUAD4C && nsub < 7) return fail(PO_ERR_QUADRATURE, "Quad4_Closed requires at least 7 grid points");
+    if (quad == PO_QUAD_TRAP && nsub < 1) return fail(PO_ERR_QUADRATURE, "trapezoid requires at least 2 slices");
+    if (quad < 0 || quad > PO_QUAD_TRAP) return fail(PO_ERR_INVALID, "unknown quadrature");
+    po_ctx *ctx = q->ctx;
+    const long M = q->sh.M;
+    double *d_phi = nullptr;
+    PO_HIP(hipMalloc((void **)&d_phi, sizeof(double) * M));
+    hipError_t e;
+    if (ctx->precision == PO_FP64)
+        e = po::launch_density<double>((const double *)q->slice_ptr(0), (const double *)qc->slice_ptr(0), len, M, quad,
+                                       nullptr, cc, q->ds, d_phi, ctx->stream);
+    else
+        e = po::launch_density<float>((const float *)q->slice_ptr(0), (const float *)qc->slice_ptr(0), len, M, quad,
+                                      nullptr, cc, q->ds, d_phi, ctx->stream);
+    if (e != hipSuccess) {
+        (void)hipFree(d_phi);
+        return fail(PO_ERR_DEVICE, std::string("density launch: ") + hipGetErrorString(e));
+    }
+    e = hipMemcpyAsync(host_phi, d_phi, sizeof(double) * M, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(d_phi);
+    if (e != hipSuccess) return fail(PO_ERR_DEVICE, std::string("density copy: ") + hipGetErrorString(e));
+    return PO_OK;
+}
+
+int po_model_create(po_ctx *ctx, const po_model_desc *desc, const double *wA0, const double *wB0, po_model **out) {
+    if (!ctx || !desc || !wA0 || !wB0 || !out) return fail(PO_ERR_INVALID, "null argument");
+    PO_HIP(hipSetDevice(ctx->device));
+    po_model *m;
+    int rc;
+    if (ctx->precision == PO_FP64) {
+        auto *t = new po::Model<double>();
+        rc = t->init(ctx, *desc, wA0, wB0);
+        m = t;
+    } else {
+        auto *t = new po::Model<float>();
+        rc = t->init(ctx, *desc, wA0, wB0);
+        m = t;
+    }
+    if (rc) {
+        delete m;
+        return rc;
+    }
+    *out = m;
+    return PO_OK;
+}
+
+int po_model_destroy(po_model *m) {
+    // hipFree in the member destructors waits for outstanding work; the
+    // context is not touched, so a model may outlive its context's handle.
+    delete m;
+    return PO_OK;
+}
+
+int po_model_update(po_model *m, int n_iter) {
+    if (!m || n_iter < 0) return fail(PO_ERR_INVALID, "bad argument");
+    return m->update(n_iter);
+}
+
+int po_model_observe(po_model *m, double *out5) {
+    if (!m || !out5) return fail(PO_ERR_INVALID, "null argument");
+    return m->observe(out5);
+}
+
+int po_model_get(po_model *m, int which, double *host_out) {
+    if (!m || !host_out) return fail(PO_ERR_INVALID, "null argument");
+    return m->get(which, host_out);
+}
+
+int po_model_relax(po_model *m, int max_iter, int min_iter, int record_interval, double thresh, int *iters, double *H,
+                   double *residual) {
+    if (!m || record_interval < 1) return fail(PO_ERR_INVALID, "bad argument");
+    int it = 0, nrec = 0;
+    double o[5] = {0, 0, 0, 0, 0};
+    while (it < max_iter) {
+        const int chunk = record_interval - (it % record_interval);
+        const int todo = chunk < max_iter - it ? chunk : max_iter - it;
+        int rc = m->update(todo);
+        if (rc) return rc;
+        it += todo;
+        if (it % record_interval != 0) break;  // max_iter reached between records
+        if ((rc = m->observe(o))) return rc;
+        ++nrec;
+        if (!std::isfinite(o[0])) break;                                   // src/scft.cc:537-543
+        if (it > min_iter && nrec > 1 && o[3] < thresh) break;             // src/scft.cc:560-568
+    }
+    if (iters) *iters = it;
+    if (H) *H = o[0];
+    if (residual) *residual = o[3];
+    return PO_OK;
+}
+
+long po_model_contour_steps(const po_model *m) { return m ? m->steps : -1; }
+
+}  // extern "C"

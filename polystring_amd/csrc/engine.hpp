@@ -1,0 +1,86 @@
+// engine.hpp -- host-side declarations shared by the HIP kernels and the C ABI.
+//
+// Device data layout (all C-order, z fastest, as the reference's blitz arrays):
+//   real field / contour slice : M = Lx*Ly*Lz values, slice s of a propagator
+//                                at q + s*M (Propagator::_qs(len,Lx,Ly,Lz),
+//                                src/Propagator.cc:16);
+//   half spectrum              : Mh = Lx*Ly*Hz complex, Hz = Lz/2+1 (Hz = 1 when
+//                                Lz = 1), kz fastest: the real-to-complex form
+//                                of the reference's C2C transform of real data.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "fft_lds.hpp"
+
+namespace po {
+
+// Per-lane work descriptor of the fused yz-plane kernel.  One workgroup
+// handles one x-plane of one lane:
+//   q   = in0 ? (inv(in0) * post0 / M) : qin
+//   q   = in1 ? (4 * inv(in1) * post1 / M - q) / 3 : q    (RQM4 combine)
+//   qout, qout2 <- q                                        (optional stores)
+//   outj = fwd(q * prej)  for j = 0, 1                      (optional)
+// inv() is the inverse y transform followed by complex-to-real along z;
+// fwd() is real-to-complex along z followed by the forward y transform.
+template <typename T> struct PlaneDesc {
+    const cx<T> *in0, *in1;
+    const T *qin;
+    const T *post0, *post1;
+    T *qout, *qout2;
+    const T *pre0, *pre1;
+    cx<T> *out0, *out1;
+};
+
+// Per-lane descriptor of the x-pencil kernel: S <- invX(K * fwdX(S)) with
+// K(kx,ky,kz) = fx[kx] * fy[ky] * fz[kz] (orthogonal cells) or tab[...] (full
+// table, general cells).
+template <typename T> struct XDesc {
+    cx<T> *S;
+    const T *fx, *fy, *fz;
+    const T *tab;
+};
+
+struct Shape {
+    int Lx, Ly, Lz, Hz;
+    long M, Mh;
+};
+
+// Device tables of N-th roots of unity exp(-2 pi i m / N), m in [0, N).
+template <typename T> struct Roots {
+    const cx<T> *x, *y, *zh, *z;  // lengths Lx, Ly, Lz/2, Lz
+};
+
+bool plane_supported(int Ly, int Lz);
+bool xpass_supported(int Lx);
+
+template <typename T>
+hipError_t launch_plane(const Shape &sh, const Roots<T> &rt, const PlaneDesc<T> *d_desc, int nlanes,
+                        hipStream_t st);
+template <typename T>
+hipError_t launch_xpass(const Shape &sh, const Roots<T> &rt, const XDesc<T> *d_desc, int nlanes,
+                        hipStream_t st);
+
+// Elementwise / reduction launchers (kernels.hip).
+template <typename T>
+hipError_t launch_exp_factor(const double *w, T *out, double coef, long M, hipStream_t st);
+template <typename T>
+hipError_t launch_fill(T *p, T v, long n, hipStream_t st);
+
+// quadrature kinds, src/Model_AB.cc:713-735
+enum Quad { QUAD_SIMPSON = 0, QUAD_QUAD4C = 1, QUAD_TRAP = 2 };
+
+// cc = 1/Qp[0] when Qp is non-null (device scalar), else ccv.
+template <typename T>
+hipError_t launch_density(const T *q, const T *qc, int len, long M, int quad, const double *Qp, double ccv,
+                          double ds, double *phi, hipStream_t st);
+// kRedBlocks partial sums of x into partials[0..kRedBlocks)
+template <typename T>
+hipError_t launch_mean_partials(const T *x, long M, double *partials, hipStream_t st);
+
+constexpr int kRedBlocks = 512;   // partial-sum blocks of the reduction kernels
+constexpr int kRedThreads = 256;
+
+}  // namespace po

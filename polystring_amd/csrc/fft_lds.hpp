@@ -1,0 +1,176 @@
+// fft_lds.hpp -- LDS-staged mixed-radix Stockham FFT building blocks for gfx950.
+//
+// A workgroup holds NP pencils of length N in LDS; element e of pencil p lives
+// at buf[p * ps + e * es] (complex, interleaved).  Every stage reads R values
+// per butterfly into registers, twiddles, applies a hand-written DFT_R, waits
+// on a barrier and writes them back in Stockham autosort order, so the
+// transform is in place in LDS and needs no bit reversal.  Stage with
+// sub-transform length L and radix R, butterfly j in [0, N/R), k = j mod L:
+//     v_r = x[j + r N/R] * W_{LR}^{r k},  v = DFT_R(v),
+//     x'[(j / L) L R + k + r L] = v_r,
+// with W_m = exp(-2 pi i / m) forward and its conjugate inverse.  Both
+// directions are unnormalised.  Twiddles come from a table of the N-th roots
+// of unity (computed on the host in long double and rounded once).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <type_traits>
+
+namespace po {
+
+template <typename T> struct vec2;
+template <> struct vec2<double> { using type = double2; };
+template <> struct vec2<float> { using type = float2; };
+template <typename T> using cx = typename vec2<T>::type;
+
+template <typename T> __device__ __forceinline__ cx<T> mk(T a, T b) { cx<T> r; r.x = a; r.y = b; return r; }
+template <typename T> __device__ __forceinline__ cx<T> add(cx<T> a, cx<T> b) { return mk<T>(a.x + b.x, a.y + b.y); }
+template <typename T> __device__ __forceinline__ cx<T> sub(cx<T> a, cx<T> b) { return mk<T>(a.x - b.x, a.y - b.y); }
+template <typename T> __device__ __forceinline__ cx<T> scl(cx<T> a, T s) { return mk<T>(a.x * s, a.y * s); }
+template <typename T> __device__ __forceinline__ cx<T> conj(cx<T> a) { return mk<T>(a.x, -a.y); }
+// a * b, or a * conj(b) when CONJ
+template <bool CONJ, typename T> __device__ __forceinline__ cx<T> cmul(cx<T> a, cx<T> b) {
+    if (CONJ) return mk<T>(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+    return mk<T>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// multiply by -i (forward quarter turn) or +i (inverse)
+template <bool INV, typename T> __device__ __forceinline__ cx<T> rot(cx<T> a) {
+    return INV ? mk<T>(-a.y, a.x) : mk<T>(a.y, -a.x);
+}
+
+// ---------------------------------------------------------------- DFT_R --
+template <int R, bool INV, typename T> struct Dft;
+
+template <bool INV, typename T> struct Dft<2, INV, T> {
+    static __device__ __forceinline__ void run(cx<T> *v) {
+        cx<T> a = v[0], b = v[1];
+        v[0] = add<T>(a, b);
+        v[1] = sub<T>(a, b);
+    }
+};
+
+template <bool INV, typename T> struct Dft<3, INV, T> {
+    static __device__ __forceinline__ void run(cx<T> *v) {
+        const T s3 = (T)0.86602540378443864676372317075293618;
+        cx<T> t1 = add<T>(v[1], v[2]);
+        cx<T> t2 = sub<T>(v[1], v[2]);
+        cx<T> m = sub<T>(v[0], scl<T>(t1, (T)0.5));
+        // forward: X1 = m - i s3 t2, X2 = m + i s3 t2
+        cx<T> it = rot<INV, T>(scl<T>(t2, s3));  // (-i or +i) * s3 * t2
+        v[0] = add<T>(v[0], t1);
+        v[1] = add<T>(m, it);
+        v[2] = sub<T>(m, it);
+    }
+};
+
+template <bool INV, typename T> struct Dft<4, INV, T> {
+    static __device__ __forceinline__ void run(cx<T> *v) {
+        cx<T> a0 = add<T>(v[0], v[2]), a1 = sub<T>(v[0], v[2]);
+        cx<T> a2 = add<T>(v[1], v[3]), a3 = rot<INV, T>(sub<T>(v[1], v[3]));
+        v[0] = add<T>(a0, a2);
+        v[2] = sub<T>(a0, a2);
+        v[1] = add<T>(a1, a3);
+        v[3] = sub<T>(a1, a3);
+    }
+};
+
+template <bool INV, typename T> struct Dft<8, INV, T> {
+    static __device__ __forceinline__ void run(cx<T> *v) {
+        const T h = (T)0.70710678118654752440084436210484904;
+        cx<T> e[4] = {v[0], v[2], v[4], v[6]};
+        cx<T> o[4] = {v[1], v[3], v[5], v[7]};
+        Dft<4, INV, T>::run(e);
+        Dft<4, INV, T>::run(o);
+        // W8^1 = (1 -/+ i)/sqrt2, W8^2 = -/+ i, W8^3 = (-1 -/+ i)/sqrt2
+        cx<T> o1 = scl<T>(add<T>(o[1], rot<INV, T>(o[1])), h);
+        cx<T> o2 = rot<INV, T>(o[2]);
+        cx<T> o3 = scl<T>(sub<T>(rot<INV, T>(o[3]), o[3]), h);
+        v[0] = add<T>(e[0], o[0]);
+        v[4] = sub<T>(e[0], o[0]);
+        v[1] = add<T>(e[1], o1);
+        v[5] = sub<T>(e[1], o1);
+        v[2] = add<T>(e[2], o2);
+        v[6] = sub<T>(e[2], o2);
+        v[3] = add<T>(e[3], o3);
+        v[7] = sub<T>(e[3], o3);
+    }
+};
+
+// ---------------------------------------------------------- radix plans --
+// Radix sequences per supported length; the first stage has L = 1.
+template <int N> struct Plan;
+template <> struct Plan<1> { static constexpr int n = 0; static constexpr int r[1] = {1}; };
+template <> struct Plan<2> { static constexpr int n = 1; static constexpr int r[1] = {2}; };
+template <> struct Plan<3> { static constexpr int n = 1; static constexpr int r[1] = {3}; };
+template <> struct Plan<4> { static constexpr int n = 1; static constexpr int r[1] = {4}; };
+template <> struct Plan<6> { static constexpr int n = 2; static constexpr int r[2] = {2, 3}; };
+template <> struct Plan<8> { static constexpr int n = 1; static constexpr int r[1] = {8}; };
+template <> struct Plan<12> { static constexpr int n = 2; static constexpr int r[2] = {4, 3}; };
+template <> struct Plan<16> { static constexpr int n = 2; static constexpr int r[2] = {4, 4}; };
+template <> struct Plan<24> { static constexpr int n = 2; static constexpr int r[2] = {8, 3}; };
+template <> struct Plan<32> { static constexpr int n = 2; static constexpr int r[2] = {8, 4}; };
+template <> struct Plan<48> { static constexpr int n = 3; static constexpr int r[3] = {4, 4, 3}; };
+template <> struct Plan<64> { static constexpr int n = 2; static constexpr int r[2] = {8, 8}; };
+template <> struct Plan<96> { static constexpr int n = 3; static constexpr int r[3] = {8, 4, 3}; };
+template <> struct Plan<128> { static constexpr int n = 3; static constexpr int r[3] = {8, 4, 4}; };
+template <> struct Plan<256> { static constexpr int n = 3; static constexpr int r[3] = {8, 8, 4}; };
+
+// One Stockham stage over NP pencils.  PFAST: consecutive threads take
+// consecutive pencils (use when pencils are adjacent in LDS); otherwise
+// consecutive butterflies of one pencil.
+template <typename T, int N, int R, int L, bool INV, int NT, int NP, bool PFAST>
+__device__ __forceinline__ void fft_stage(cx<T> *buf, int ps, int es, const cx<T> *__restrict__ roots) {
+    constexpr int NB = N / R;
+    constexpr int TOT = NP * NB;
+    constexpr int IT = (TOT + NT - 1) / NT;
+    constexpr int TS = N / (L * R);  // W_{LR}^{e} = W_N^{e * TS}
+    cx<T> v[IT][R];
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int b = tid + it * NT;
+        if ((TOT % NT == 0) || b < TOT) {
+            const int p = PFAST ? b % NP : b / NB;
+            const int j = PFAST ? b / NP : b % NB;
+            const cx<T> *src = buf + p * ps;
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[it][r] = src[(j + r * NB) * es];
+            if (L > 1) {
+                const int k = j % L;
+#pragma unroll
+                for (int r = 1; r < R; ++r) v[it][r] = cmul<INV, T>(v[it][r], roots[r * k * TS]);
+            }
+            Dft<R, INV, T>::run(v[it]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int b = tid + it * NT;
+        if ((TOT % NT == 0) || b < TOT) {
+            const int p = PFAST ? b % NP : b / NB;
+            const int j = PFAST ? b / NP : b % NB;
+            const int k = j % L;
+            cx<T> *dst = buf + p * ps + ((j / L) * L * R + k) * es;
+#pragma unroll
+            for (int r = 0; r < R; ++r) dst[r * L * es] = v[it][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <typename T, int N, int S, int L, bool INV, int NT, int NP, bool PFAST>
+__device__ __forceinline__ void fft_stages(cx<T> *buf, int ps, int es, const cx<T> *roots) {
+    if constexpr (S < Plan<N>::n) {
+        constexpr int R = Plan<N>::r[S];
+        fft_stage<T, N, R, L, INV, NT, NP, PFAST>(buf, ps, es, roots);
+        fft_stages<T, N, S + 1, L * R, INV, NT, NP, PFAST>(buf, ps, es, roots);
+    }
+}
+
+// Full in-place length-N transform of NP pencils; ends with a barrier.
+template <typename T, int N, bool INV, int NT, int NP, bool PFAST>
+__device__ __forceinline__ void fft_lds(cx<T> *buf, int ps, int es, const cx<T> *roots) {
+    if constexpr (N > 1) fft_stages<T, N, 0, 1, INV, NT, NP, PFAST>(buf, ps, es, roots);
+}
+
+}  // namespace po

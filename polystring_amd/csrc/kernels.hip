@@ -1,0 +1,364 @@
+// kernels.hip -- pointwise, contour-quadrature, reduction and Anderson-mixing
+// kernels.  All are HBM-streaming kernels: grid-stride loops over the M grid
+// points, FP64 accumulation, block partial sums reduced in a fixed order so
+// every result is deterministic run to run.
+#include <cmath>
+
+#include "anderson.hpp"
+#include "engine.hpp"
+
+namespace po {
+
+__device__ __forceinline__ double block_sum(double v, double *sh) {
+    // wave64 shuffle tree, then one value per wave through LDS
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0) {
+        const int nw = (blockDim.x + 63) >> 6;
+        for (int i = 0; i < nw; ++i) s += sh[i];
+    }
+    return s;  // valid in thread 0
+}
+
+static inline int grid_for(long n, int threads, int cap) {
+    long g = (n + threads - 1) / threads;
+    if (g > cap) g = cap;
+    if (g < 1) g = 1;
+    return (int)g;
+}
+
+// ------------------------------------------------------------- factors --
+// out = exp(coef * w); coef carries the sign, as in exp(-0.5 * ds * w)
+// (src/RQM4.cc:50,52, src/PseudoSpectral.cc:78).
+template <typename T>
+__global__ void k_exp_factor(const double *__restrict__ w, T *__restrict__ out, double coef, long M) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < M; i += (long)gridDim.x * blockDim.x)
+        out[i] = (T)exp(coef * w[i]);
+}
+
+template <typename T>
+hipError_t launch_exp_factor(const double *w, T *out, double coef, long M, hipStream_t st) {
+    hipLaunchKernelGGL(k_exp_factor<T>, dim3(grid_for(M, 256, 8192)), dim3(256), 0, st, w, out, coef, M);
+    return hipGetLastError();
+}
+
+template <typename T> __global__ void k_fill(T *p, T v, long n) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = v;
+}
+
+template <typename T> hipError_t launch_fill(T *p, T v, long n, hipStream_t st) {
+    hipLaunchKernelGGL(k_fill<T>, dim3(grid_for(n, 256, 8192)), dim3(256), 0, st, p, v, n);
+    return hipGetLastError();
+}
+
+// ----------------------------------------------------------- quadrature --
+// phi(r) = scale * sum_s c_s q[s](r) qc[n-s](r):
+//   Simpson      src/Simpson.cc:31-44      (qq0 + qqn + 4 sum_odd + 2 sum_even) * cc ds / 3
+//   Quad4_Closed src/Quadrature4.cc:26-52  weights 9/24, 28/24, 23/24, 1, ..., 1, 23/24, 28/24, 9/24
+//   trapezoid    src/Density.cc:107-121    sum - (qq0 + qqn)/2
+// scale is cc*ds/3 (Simpson) or cc*ds, with cc = 1/Q read from the device
+// when Qp is given (src/Model_AB.cc:195-199) or ccv otherwise.
+template <typename T>
+__global__ void k_density(const T *__restrict__ q, const T *__restrict__ qc, int len, long M, int quad,
+                          const double *__restrict__ Qp, double ccv, double ds, double *__restrict__ phi) {
+    const int n = len - 1;
+    const double cc = Qp ? 1.0 / Qp[0] : ccv;
+    const double scale = quad == QUAD_SIMPSON ? cc * ds / 3.0 : cc * ds;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < M; i += (long)gridDim.x * blockDim.x) {
+        auto qq = [&](int s) { return (double)q[(long)s * M + i] * (double)qc[(long)(n - s) * M + i]; };
+        double v;
+        if (quad == QUAD_SIMPSON) {
+            double s1 = 0.0, s2 = 0.0;
+            for (int s = 1; s <= n - 1; s += 2) s1 += 4.0 * qq(s);
+            for (int s = 2; s <= n - 2; s += 2) s2 += 2.0 * qq(s);
+            v = (qq(0) + qq(n)) + s1 + s2;
+        } else if (quad == QUAD_QUAD4C) {
+            const double c0 = 0.375, c1 = 7 / 6.0, c2 = 23 / 24.0;
+            v = c0 * qq(0) + c1 * qq(1) + c2 * qq(2);
+            double mid = 0.0;
+            for (int s = 3; s <= n - 3; ++s) mid += qq(s);
+            v += mid;
+            v += c2 * qq(n - 2) + c1 * qq(n - 1) + c0 * qq(n);
+        } else {
+            double sum = 0.0;
+            for (int s = 0; s <= n; ++s) sum += qq(s);
+            const double q0 = 0.5 * (double)q[i] * (double)qc[(long)n * M + i];
+            const double qt = 0.5 * (double)qc[i] * (double)q[(long)n * M + i];
+            v = sum - (q0 + qt);
+        }
+        phi[i] = v * scale;
+    }
+}
+
+template <typename T>
+hipError_t launch_density(const T *q, const T *qc, int len, long M, int quad, const double *Qp, double cc,
+                          double ds, double *phi, hipStream_t st) {
+    hipLaunchKernelGGL(k_density<T>, dim3(grid_for(M, 256, 8192)), dim3(256), 0, st, q, qc, len, M, quad, Qp, cc,
+                       ds, phi);
+    return hipGetLastError();
+}
+
+// ----------------------------------------------------------- reductions --
+template <typename T>
+__global__ void __launch_bounds__(kRedThreads) k_sum_partials(const T *__restrict__ x, long M, double *partials) {
+    __shared__ double sh[kRedThreads / 64];
+    double s = 0.0;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < M; i += (long)gridDim.x * blockDim.x)
+        s += (double)x[i];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+template <typename T> hipError_t launch_mean_partials(const T *x, long M, double *partials, hipStream_t st) {
+    hipLaunchKernelGGL(k_sum_partials<T>, dim3(kRedBlocks), dim3(kRedThreads), 0, st, x, M, partials);
+    return hipGetLastError();
+}
+
+// out[v] = (sum_b partials[b * nval + v]) / div, summed in block order; with
+// div = M this is blitz::mean (src/Grid.cc:329-338).
+__global__ void k_finish_partials(const double *__restrict__ partials, int nblk, int nval, double div, double *out) {
+    for (int v = threadIdx.x; v < nval; v += blockDim.x) {
+        double s = 0.0;
+        for (int b = 0; b < nblk; ++b) s += partials[(long)b * nval + v];
+        out[v] = s / div;
+    }
+}
+
+hipError_t launch_finish_partials(const double *partials, int nblk, int nval, double div, double *out,
+                                  hipStream_t st) {
+    hipLaunchKernelGGL(k_finish_partials, dim3(1), dim3(64), 0, st, partials, nblk, nval, div, out);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------- Anderson --
+// d = (chiN phi_other + yita) - w with yita = (wA + wB - chiN)/2
+// (src/Model_AB.cc:232-238, src/FieldAX.cc:311-313), stored into the ring
+// slot cur; per-block partial sums of the Gram row  sum d_cur * d_{k-j}
+// (src/FieldAX.cc:333-341) for both species and of w^2 (the residual
+// denominator, src/FieldAX.cc:106-116).  Partial layout per block:
+// [GA(0..nn), GB(0..nn), WA2, WB2].
+template <int NN>
+__global__ void __launch_bounds__(kRedThreads)
+k_anderson_d(AndersonArgs a, const double *__restrict__ phiA, const double *__restrict__ phiB,
+             const double *__restrict__ wA, const double *__restrict__ wB, double *partials) {
+    __shared__ double sh[kRedThreads / 64];
+    constexpr int NV = 2 * (NN + 1) + 2;
+    double acc[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+    double *dAc = a.dA + (long)a.cur * a.M;
+    double *dBc = a.dB + (long)a.cur * a.M;
+    const double *hA[NN + 1], *hB[NN + 1];
+#pragma unroll
+    for (int j = 1; j <= NN; ++j) {
+        hA[j] = a.dA + (long)a.hidx[j] * a.M;
+        hB[j] = a.dB + (long)a.hidx[j] * a.M;
+    }
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < a.M; i += (long)gridDim.x * blockDim.x) {
+        const double wa = wA[i], wb = wB[i];
+        const double yita = 0.5 * ((wa + wb) - a.chiN);
+        const double da = (a.chiN * phiB[i] + yita) - wa;
+        const double db = (a.chiN * phiA[i] + yita) - wb;
+        dAc[i] = da;
+        dBc[i] = db;
+        acc[0] += da * da;
+        acc[NN + 1] += db * db;
+#pragma unroll
+        for (int j = 1; j <= NN; ++j) {
+            acc[j] += da * hA[j][i];
+            acc[NN + 1 + j] += db * hB[j][i];
+        }
+        acc[NV - 2] += wa * wa;
+        acc[NV - 1] += wb * wb;
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        const double s = block_sum(acc[v], sh);
+        if (threadIdx.x == 0) partials[(long)blockIdx.x * NV + v] = s;
+    }
+}
+
+// Single-workgroup step: finish the means, shift and refill B for both
+// species (src/FieldAX.cc:321-345), form U = UA + UB, V = VA + VB
+// (src/FieldAX.cc:352-381, src/Model_AB.cc:239-249), solve U C = V by LU with
+// partial pivoting (arma::solve -> dgesv), and record the residual
+// (src/Model_AB.cc:309-315).
+__global__ void k_anderson_solve(AndersonArgs a, const double *__restrict__ partials, int nblk,
+                                 double *__restrict__ BA, double *__restrict__ BB, double *__restrict__ C,
+                                 double *__restrict__ obs) {
+    __shared__ double sums[2 * (kMaxHist + 1) + 2];
+    __shared__ double U[kMaxHist * kMaxHist], V[kMaxHist];
+    const int nv = 2 * (a.nn + 1) + 2;
+    const int n1 = a.n + 1;
+    for (int v = threadIdx.x; v < nv; v += blockDim.x) {
+        double s = 0.0;
+        for (int b = 0; b < nblk; ++b) s += partials[(long)b * nv + v];
+        sums[v] = s / (double)a.M;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int sp = 0; sp < 2; ++sp) {
+            double *B = sp ? BB : BA;
+            for (int i = a.n; i >= 1; --i)
+                for (int j = a.n; j >= i; --j) B[i * n1 + j] = B[(i - 1) * n1 + (j - 1)];
+            for (int j = 0; j <= a.nn; ++j) B[j] = sums[sp * (a.nn + 1) + j];
+            for (int i = 0; i <= a.n; ++i)
+                for (int j = i + 1; j <= a.n; ++j) B[j * n1 + i] = B[i * n1 + j];
+        }
+        const int nn = a.nn;
+        for (int i = 0; i < nn; ++i) {
+            for (int j = 0; j < nn; ++j) {
+                double u = 0.0;
+                for (int sp = 0; sp < 2; ++sp) {
+                    const double *B = sp ? BB : BA;
+                    const double ui = B[0] - B[(i + 1) * n1] - B[j + 1] + B[(i + 1) * n1 + (j + 1)];
+                    u = sp ? u + ui : ui;
+                }
+                U[i * nn + j] = u;
+            }
+            V[i] = (BA[0] - BA[(i + 1) * n1]) + (BB[0] - BB[(i + 1) * n1]);
+        }
+        int ok = 1;
+        for (int k = 0; k < nn && ok; ++k) {
+            int p = k;
+            double best = fabs(U[k * nn + k]);
+            for (int i = k + 1; i < nn; ++i)
+                if (fabs(U[i * nn + k]) > best) { best = fabs(U[i * nn + k]); p = i; }
+            if (best == 0.0) { ok = 0; break; }
+            if (p != k) {
+                for (int j = 0; j < nn; ++j) { double t = U[k * nn + j]; U[k * nn + j] = U[p * nn + j]; U[p * nn + j] = t; }
+                double t = V[k]; V[k] = V[p]; V[p] = t;
+            }
+            for (int i = k + 1; i < nn; ++i) {
+                const double f = U[i * nn + k] / U[k * nn + k];
+                U[i * nn + k] = f;
+                for (int j = k + 1; j < nn; ++j) U[i * nn + j] -= f * U[k * nn + j];
+                V[i] -= f * V[k];
+            }
+        }
+        for (int i = nn - 1; i >= 0; --i) {
+            double s = V[i];
+            for (int j = i + 1; j < nn; ++j) s -= U[i * nn + j] * C[j];
+            C[i] = ok ? s / U[i * nn + i] : 0.0;
+        }
+        for (int i = 0; i < nn && !ok; ++i) C[i] = 0.0;
+        obs[OBS_RESIDUAL] = sqrt((sums[0] + sums[a.nn + 1]) / (sums[nv - 2] + sums[nv - 1]));
+        obs[OBS_SINGULAR] = ok ? 0.0 : 1.0;
+    }
+}
+
+// w_new = W + lam D with W = w_k + sum_i C_{i-1} (w_{k-i} - w_k) and D alike
+// (src/FieldAX.cc:286-305,395-427), for both species; the new field goes to
+// ring slot next and to the current-field buffer.  Fused: the Boltzmann
+// factor tables of the next propagator solve and the partial sums of
+// Hw = <chiN phiA phiB - wA phiA - wB phiB> (src/Model_AB.cc:280-284).
+template <typename T>
+__global__ void __launch_bounds__(kRedThreads)
+k_anderson_mix(AndersonArgs a, const double *__restrict__ C, double *__restrict__ wA, double *__restrict__ wB,
+               const double *__restrict__ phiA, const double *__restrict__ phiB, FactorOut<T> fo,
+               double *partials) {
+    __shared__ double sh[kRedThreads / 64];
+    double hw = 0.0;
+    const double *wAc = a.wA + (long)a.cur * a.M, *wBc = a.wB + (long)a.cur * a.M;
+    const double *dAc = a.dA + (long)a.cur * a.M, *dBc = a.dB + (long)a.cur * a.M;
+    double *wAn = a.wA + (long)a.next * a.M, *wBn = a.wB + (long)a.next * a.M;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < a.M; i += (long)gridDim.x * blockDim.x) {
+        const double wa = wAc[i], wb = wBc[i], da = dAc[i], db = dBc[i];
+        double WA = wa, WB = wb, DA = da, DB = db;
+        for (int j = 1; j <= a.nc; ++j) {
+            const long off = (long)a.hidx[j] * a.M + i;
+            const double c = C[j - 1];
+            WA += c * (a.wA[off] - wa);
+            WB += c * (a.wB[off] - wb);
+            DA += c * (a.dA[off] - da);
+            DB += c * (a.dB[off] - db);
+        }
+        const double na = WA + a.lamA * DA;
+        const double nb = WB + a.lamB * DB;
+        wAn[i] = na;
+        wBn[i] = nb;
+        wA[i] = na;
+        wB[i] = nb;
+        if (fo.eA2) {
+            fo.eA2[i] = (T)exp(fo.cA2 * na);
+            fo.eA4[i] = (T)exp(fo.cA4 * na);
+            fo.eB2[i] = (T)exp(fo.cB2 * nb);
+            fo.eB4[i] = (T)exp(fo.cB4 * nb);
+        }
+        if (phiA) {
+            const double pa = phiA[i], pb = phiB[i];
+            hw += ((a.chiN * pa) * pb - na * pa) - nb * pb;
+        }
+    }
+    const double s = block_sum(hw, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+// H = Hw + Hs with Hs = -log Q (src/Model_AB.cc:294-300)
+__global__ void k_observables(const double *__restrict__ partials, int nblk, long M, double *obs) {
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int b = 0; b < nblk; ++b) s += partials[b];
+        const double hw = s / (double)M;
+        const double hs = -log(obs[OBS_Q]);
+        obs[OBS_HW] = hw;
+        obs[OBS_HS] = hs;
+        obs[OBS_H] = hw + hs;
+    }
+}
+
+template <int NN>
+static hipError_t anderson_d_nn(const AndersonArgs &a, const double *phiA, const double *phiB, const double *wA,
+                                const double *wB, double *partials, hipStream_t st) {
+    if (a.nn == NN) {
+        hipLaunchKernelGGL(k_anderson_d<NN>, dim3(kRedBlocks), dim3(kRedThreads), 0, st, a, phiA, phiB, wA, wB,
+                           partials);
+        return hipGetLastError();
+    }
+    if constexpr (NN < kMaxHist) return anderson_d_nn<NN + 1>(a, phiA, phiB, wA, wB, partials, st);
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_anderson_d(const AndersonArgs &a, const double *phiA, const double *phiB, const double *wA,
+                             const double *wB, double *partials, hipStream_t st) {
+    return anderson_d_nn<0>(a, phiA, phiB, wA, wB, partials, st);
+}
+
+hipError_t launch_anderson_solve(const AndersonArgs &a, const double *partials, double *BA, double *BB, double *C,
+                                 double *obs, hipStream_t st) {
+    hipLaunchKernelGGL(k_anderson_solve, dim3(1), dim3(64), 0, st, a, partials, kRedBlocks, BA, BB, C, obs);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_anderson_mix(const AndersonArgs &a, const double *C, double *wA, double *wB, const double *phiA,
+                               const double *phiB, const FactorOut<T> &fo, double *partials, hipStream_t st) {
+    hipLaunchKernelGGL(k_anderson_mix<T>, dim3(kRedBlocks), dim3(kRedThreads), 0, st, a, C, wA, wB, phiA, phiB, fo,
+                       partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_observables(const double *partials, long M, double *obs, hipStream_t st) {
+    hipLaunchKernelGGL(k_observables, dim3(1), dim3(64), 0, st, partials, kRedBlocks, M, obs);
+    return hipGetLastError();
+}
+
+#define PO_INST(T)                                                                                              \
+    template hipError_t launch_exp_factor<T>(const double *, T *, double, long, hipStream_t);                   \
+    template hipError_t launch_fill<T>(T *, T, long, hipStream_t);                                              \
+    template hipError_t launch_density<T>(const T *, const T *, int, long, int, const double *, double, double, \
+                                          double *, hipStream_t);                                               \
+    template hipError_t launch_mean_partials<T>(const T *, long, double *, hipStream_t);                        \
+    template hipError_t launch_anderson_mix<T>(const AndersonArgs &, const double *, double *, double *,        \
+                                               const double *, const double *, const FactorOut<T> &, double *,  \
+                                               hipStream_t);
+PO_INST(double)
+PO_INST(float)
+template hipError_t launch_fill<double2>(double2 *, double2, long, hipStream_t);
+template hipError_t launch_fill<float2>(float2 *, float2, long, hipStream_t);
+
+}  // namespace po

@@ -1,0 +1,113 @@
+"""Synthetic SCFT workloads of the benchmark configurations.
+
+The reference reads initial fields from a MAT file (gridInitType = File,
+src/Grid.cc:694-711) and ships no gyroid / cylinder templates (its
+GYROID_PATTERN is declared but not implemented, src/Helper.cc:28-52), so the
+templates below are defined here.  Noise is Philox4x32-10 (numpy's
+implementation) + normal sampling, seeded per configuration and per field
+(stream 0 = wA, stream 1 = wB), so CPU and GPU runs see identical bits.
+
+Ms follows the fork's convention (src/Model_AB.cc:100-111): LA = round(f Ns),
+sA = LA + 1, sB = Ns - LA + 1, ds = 1/Ns.
+"""
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+
+def noise(seed, stream, shape, sigma):
+    g = np.random.Generator(np.random.Philox(np.random.SeedSequence([int(seed), int(stream)])))
+    return sigma * g.standard_normal(shape)
+
+
+def ms_from(f, Ns):
+    LA = int(math.floor(f * Ns + 0.5))
+    return LA + 1, Ns - LA + 1
+
+
+def coords(shape, lengths):
+    """Physical coordinates x_i = i * l / L of the C-order grid."""
+    axes = [np.arange(n) * (l / n) for n, l in zip(shape, lengths)]
+    return np.meshgrid(*axes, indexing="ij")
+
+
+def gyroid(shape, lengths):
+    X, Y, Z = [2 * np.pi * c / l for c, l in zip(coords(shape, lengths), lengths)]
+    return np.sin(X) * np.cos(Y) + np.sin(Y) * np.cos(Z) + np.sin(Z) * np.cos(X)
+
+
+def hex_cylinders(shape, lengths):
+    """Three first-shell plane waves in the x-y plane (hexagonal cylinder template)."""
+    X, Y, _ = coords(shape, lengths)
+    kx, ky = 2 * np.pi / lengths[0], 2 * np.pi / lengths[1]
+    return (np.cos(kx * X) + np.cos(0.5 * kx * X + ky * Y) + np.cos(0.5 * kx * X - ky * Y)) / 1.5
+
+
+def lamellar(shape, lengths):
+    X = coords(shape, lengths)[0]
+    return np.cos(2 * np.pi * X / lengths[0])
+
+
+@dataclass
+class Workload:
+    name: str
+    shape: tuple
+    cell: tuple          # (crystal, a, b, c)
+    Ns: int
+    f: float
+    chiN: float
+    wA: np.ndarray = field(repr=False)
+    wB: np.ndarray = field(repr=False)
+    lam: tuple = (0.1, 0.1)
+    n_anderson: int = 20
+
+    @property
+    def Ms(self):
+        return ms_from(self.f, self.Ns)
+
+    def model_kwargs(self):
+        return dict(cell=self.cell, Ms=self.Ms, chiN=self.chiN, lam=self.lam, n_anderson=self.n_anderson)
+
+
+def cfg1(L=32):
+    """AB melt f=0.3, chiN=20, cubic a=4.0, Ns=100, noisy homogeneous start."""
+    shape = (L, L, L)
+    f, chiN = 0.3, 20.0
+    wA = chiN * (1 - f) + noise(1001, 0, shape, 0.5)
+    wB = chiN * f + noise(1001, 1, shape, 0.5)
+    return Workload("cfg1", shape, ("CUBIC", 4.0, 4.0, 4.0), 100, f, chiN, wA, wB)
+
+
+def cfg2(L=128, Ns=200):
+    """AB melt f=0.35, chiN=18, cubic a=4.2, 128^3, Ns=200, gyroid template."""
+    shape = (L, L, L)
+    f, chiN, a = 0.35, 18.0, 4.2
+    g = gyroid(shape, (a, a, a))
+    wA = chiN * (1 - f) + 2 * g + noise(1002, 0, shape, 0.2)
+    wB = chiN * f - 2 * g + noise(1002, 1, shape, 0.2)
+    return Workload("cfg2", shape, ("CUBIC", a, a, a), Ns, f, chiN, wA, wB)
+
+
+def cfg4(L=64):
+    """AB melt f=0.3, chiN=20, orthorhombic 3.8 x 4.0 x 4.3, 64^3, Ns=200, hex cylinders."""
+    shape = (L, L, L)
+    f, chiN = 0.3, 20.0
+    lengths = (3.8, 4.0, 4.3)
+    h = hex_cylinders(shape, lengths)
+    wA = chiN * (1 - f) + 2 * h + noise(1004, 0, shape, 0.1)
+    wB = chiN * f - 2 * h + noise(1004, 1, shape, 0.1)
+    return Workload("cfg4", shape, ("ORTHORHOMBIC",) + lengths, 200, f, chiN, wA, wB)
+
+
+def contour_steps_per_iteration(Ns):
+    """Counted contour steps per SCFT iteration: qA, qB, qBc, qAc = 2 Ns."""
+    return 2 * Ns
+
+
+def algorithmic_bytes_per_step(shape, p):
+    """(13 M + 12 Mh) p: each FFT one HBM round trip, pointwise factors fused."""
+    Lx, Ly, Lz = shape
+    M = Lx * Ly * Lz
+    Mh = Lx * Ly * (Lz // 2 + 1)
+    return (13 * M + 12 * Mh) * p

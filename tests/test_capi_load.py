@@ -1,0 +1,57 @@
+"""CPU checks of the boundary and the host logic (no GPU calls)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+
+from polystring_amd import capi, workloads
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "polystring_amd.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(po_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(capi.LIB_PATH)
+    names = declared_symbols()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(lib, n), n
+    assert sorted(capi.EXPORTS) == names
+
+
+def test_binding_loads_and_reports_version():
+    lib = capi.load()
+    assert lib.po_version() >= 100
+    assert lib.po_shape_supported(128, 128, 128) == 1
+    assert lib.po_shape_supported(64, 1, 1) == 1
+    assert lib.po_shape_supported(100, 100, 100) == 0
+
+
+def test_struct_layouts_match_header():
+    # po_grid_desc: 5 ints then 6 doubles; po_model_desc embeds it
+    assert ctypes.sizeof(capi.GridDesc) == 5 * 4 + 4 + 6 * 8
+    assert capi.ModelDesc.sA.offset == ctypes.sizeof(capi.GridDesc)
+
+
+def test_ms_and_bytes_model():
+    assert workloads.ms_from(0.3, 100) == (31, 71)
+    assert workloads.ms_from(0.35, 200) == (71, 131)
+    assert workloads.ms_from(0.3, 200) == (61, 141)
+    assert workloads.ms_from(0.33, 150) == (51, 101)
+    # (13 M + 12 Mh) p at 128^3
+    assert workloads.algorithmic_bytes_per_step((128, 128, 128), 8) == 320_339_968
+    assert workloads.algorithmic_bytes_per_step((128, 128, 128), 4) == 160_169_984
+    assert workloads.algorithmic_bytes_per_step((64, 64, 64), 8) == 40_239_104
+
+
+def test_workload_inputs_are_seeded():
+    a, b = workloads.cfg1(16), workloads.cfg1(16)
+    assert np.array_equal(a.wA, b.wA) and np.array_equal(a.wB, b.wB)
+    assert abs(a.wA.mean() - 14.0) < 0.1 and abs(a.wA.std() - 0.5) < 0.05
+    g = workloads.gyroid((16, 16, 16), (4.2, 4.2, 4.2))
+    assert abs(g.mean()) < 1e-12 and abs(g.max() - 1.5) < 0.1

@@ -1,0 +1,184 @@
+"""GPU parity: the HIP path (through the C ABI) against the FP64 oracle.
+
+Bars: FP64 propagators and densities to 1e-12 relative (max-norm), FP64 SCFT
+iterations to 1e-10 relative on H, log Q, phi and w; FP32-bulk to |dH/H| <= 1e-6.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from polystring_amd import capi, workloads
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def smooth_field(shape, seed, amp=3.0):
+    rng = np.random.default_rng(seed)
+    w = rng.standard_normal(shape)
+    # low-pass in k so the field resembles an SCFT field
+    W = np.fft.fftn(w)
+    k2 = sum(np.meshgrid(*[np.minimum(np.arange(n), n - np.arange(n)) ** 2 for n in shape], indexing="ij"))
+    W *= np.exp(-0.05 * k2)
+    w = np.real(np.fft.ifftn(W))
+    return amp * w / (np.abs(w).max() + 1e-300) + 1.0
+
+
+def relerr(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+CASES = [
+    ((32, 32, 32), ("CUBIC", 4.0), 3),
+    ((64, 1, 1), ("LAMELLAR", 4.0), 1),
+    ((32, 32, 1), ("SQUARE", 4.0), 2),
+    ((48, 48, 48), ("CUBIC", 3.7), 3),
+    ((16, 16, 16), ("ORTHORHOMBIC", 3.1, 3.5, 4.2), 3),
+]
+
+
+@pytest.mark.parametrize("shape,cell,dim", CASES)
+@pytest.mark.parametrize("algo", ["RQM4", "OS2"])
+def test_propagator_matches_oracle(ctx64, shape, cell, dim, algo):
+    print(f"\n[prop] {algo} shape={shape} cell={cell}", flush=True)
+    w = smooth_field(shape, 7)
+    length, ds = 11, 0.01
+    head = 1.0 + 0.1 * smooth_field(shape, 8, amp=1.0)
+    l, g = O.cell_shape(dim, cell[0], *(list(cell[1:]) + [0.0] * (6 - len(cell[1:]))))
+    if algo == "RQM4":
+        ref = O.rqm4_solve(O.k2_orthogonal(dim, l, shape), ds, w, head, length)
+    else:
+        ref = O.os2_solve(O.klaplacian(g, shape, ds), ds, w, head, length)
+    p = capi.Propagator(ctx64, shape, cell, length, ds, algo=algo, dim=dim)
+    p.set_head(head)
+    p.update(w)
+    got = p.qs()
+    assert relerr(got, ref) < 1e-12
+    assert abs(p.Qt() - ref[-1].mean()) <= 1e-12 * abs(ref[-1].mean())
+
+
+def test_os2_general_cell_matches_oracle(ctx64):
+    """Hexagonal 3-D cell: OS2 takes the full 27-image k^2 table."""
+    shape, dim = (32, 32, 32), 3
+    cell = ("HEXAGONAL", 4.0, 4.0, 3.5)
+    w = smooth_field(shape, 11)
+    l, g = O.cell_shape(dim, "HEXAGONAL", 4.0, 4.0, 3.5)
+    ref = O.os2_solve(O.klaplacian(g, shape, 0.02), 0.02, w, np.ones(shape), 9)
+    p = capi.Propagator(ctx64, shape, cell, 9, 0.02, algo="OS2", dim=dim)
+    p.set_head(np.ones(shape))
+    p.update(w)
+    assert relerr(p.qs(), ref) < 1e-12
+
+
+def test_rqm4_sech_case(ctx64):
+    """tests/test_rqm4.cc:21-74: 1-D, Lx=32, lx=10, Ns=101, w = 1 - 2 sech^2(0.75(2x - lx))."""
+    Lx, lx, Ns = 32, 10.0, 101
+    ds = 1.0 / (Ns - 1)
+    x = np.linspace(lx / Lx, lx, Lx)
+    w = (1.0 - 2.0 / np.cosh(0.75 * (2.0 * x - lx)) ** 2).reshape(Lx, 1, 1)
+    l, g = O.cell_shape(1, "LAMELLAR", lx)
+    ref_r = O.rqm4_solve(O.k2_orthogonal(1, l, (Lx, 1, 1)), ds, w, np.ones((Lx, 1, 1)), Ns)
+    ref_o = O.os2_solve(O.klaplacian(g, (Lx, 1, 1), ds), ds, w, np.ones((Lx, 1, 1)), Ns)
+    for algo, ref in (("RQM4", ref_r), ("OS2", ref_o)):
+        p = capi.Propagator(ctx64, (Lx, 1, 1), ("LAMELLAR", lx), Ns, ds, algo=algo, dim=1)
+        p.set_head(np.ones((Lx, 1, 1)))
+        p.update(w)
+        assert relerr(p.get_tail(), ref[-1]) < 1e-12
+    # cross-scheme check of the reference test: RQM4 and OS2 tails agree to O(ds^2)
+    assert np.abs(ref_r[-1] - ref_o[-1]).max() < 1e-3
+
+
+@pytest.mark.parametrize("quad,length", [("simpson", 11), ("quad4", 12), ("trap", 9)])
+def test_density_matches_oracle(ctx64, quad, length):
+    shape = (32, 32, 32)
+    ds = 0.01
+    w = smooth_field(shape, 3)
+    q = capi.Propagator(ctx64, shape, ("CUBIC", 4.0), length, ds)
+    qc = capi.Propagator(ctx64, shape, ("CUBIC", 4.0), length, ds)
+    q.set_head(np.ones(shape))
+    qc.set_head(1.0 + 0.2 * smooth_field(shape, 4, amp=1.0))
+    q.update(w)
+    qc.update(w + 0.5)
+    got = capi.density(q, qc, 0.7, quad)
+    ref = O.quadrature(quad, q.qs(), qc.qs(), 0.7, ds)
+    assert relerr(got, ref) < 1e-13
+
+
+def test_density_rejects_bad_slice_count(ctx64):
+    shape = (16, 16, 16)
+    q = capi.Propagator(ctx64, shape, ("CUBIC", 4.0), 4, 0.01)
+    with pytest.raises(capi.POError) as e:
+        capi.density(q, q, 1.0, "simpson")   # n = 3 is odd (src/Simpson.cc:19-22)
+    assert e.value.code == capi.PO_ERR_QUADRATURE
+    with pytest.raises(capi.POError):
+        capi.density(q, q, 1.0, "quad4")     # n < 7 (src/Quadrature4.cc:22-25)
+
+
+def _compare_models(gm, om, tol):
+    o = gm.observe()
+    assert abs(o["H"] - om.H()) <= tol * abs(om.H())
+    assert abs(o["Q"] - om.Q()) <= tol * abs(om.Q())
+    assert abs(o["residual"] - om.residual()) <= max(tol * 1e3 * om.residual(), 1e-14)
+    for f in ("wA", "wB", "phiA", "phiB"):
+        assert relerr(gm.get(f), om.get(f)) <= tol, f
+
+
+@pytest.mark.parametrize("algo", ["RQM4", "OS2"])
+def test_model_iterations_match_oracle(ctx64, algo):
+    """cfg1 system (32^3, Ns=100, f=0.3, chiN=20), 6 Anderson iterations."""
+    wl = workloads.cfg1()
+    kw = wl.model_kwargs()
+    kw["n_anderson"] = 4
+    om = O.Model(wl.wA, wl.wB, dim=3, mde=algo, **kw)
+    gm = capi.ModelAB(ctx64, wl.wA, wl.wB, algo=algo, **kw)
+    for it in range(6):
+        om.update()
+        gm.update()
+        print(f"  it {it}: H gpu {gm.observe()['H']!r} oracle {om.H()!r}", flush=True)
+        _compare_models(gm, om, 1e-10)
+
+
+def test_quad4_branch_matches_oracle(ctx64):
+    """Even slice counts select Quad4_Closed (src/Model_AB.cc:719-728)."""
+    shape = (16, 16, 16)
+    wl = workloads.cfg1(16)
+    kw = dict(cell=("CUBIC", 4.0), Ms=(30, 72), chiN=20.0, lam=(0.1, 0.1), n_anderson=3)
+    om = O.Model(wl.wA, wl.wB, dim=3, **kw)
+    gm = capi.ModelAB(ctx64, wl.wA, wl.wB, **kw)
+    for _ in range(4):
+        om.update()
+        gm.update()
+    _compare_models(gm, om, 1e-10)
+
+
+def test_lam1d_published_free_energy(ctx64):
+    """polyorder-master example/AB/LAM/1D: RQM4 + Anderson converges to H = 4.778413065983450."""
+    known = json.load(open(os.path.join(GOLD, "known_answers.json")))
+    f = np.load(os.path.join(GOLD, "lam1d_field.npz"))
+    gm = capi.ModelAB(ctx64, f["wA"], f["wB"], cell=("LAMELLAR", 4.0), Ms=(51, 51), chiN=30.0,
+                      lam=(0.05, 0.05), n_anderson=5, dim=1)
+    it, H, res = gm.relax(2000, 10, 1, 1e-10)
+    print(f"\n[lam1d] iterations {it} H {H!r} residual {res:.3e}", flush=True)
+    Hpub = known["1D"]["RQM4+Anderson"]["H"]
+    assert res < 1e-10
+    assert abs(H - Hpub) <= 1e-8 * Hpub
+    phiA = gm.get("phiA")
+    assert abs(phiA.mean() - 0.5) < 1e-12
+
+
+def test_mixed_precision_error(ctx32):
+    """FP32 bulk vs the FP64 oracle on the cfg1 system: |dH/H| <= 1e-6 after 6 iterations."""
+    wl = workloads.cfg1()
+    kw = wl.model_kwargs()
+    om = O.Model(wl.wA, wl.wB, dim=3, **kw)
+    gm = capi.ModelAB(ctx32, wl.wA, wl.wB, **kw)
+    for _ in range(6):
+        om.update()
+        gm.update()
+    o = gm.observe()
+    dH = abs(o["H"] - om.H()) / abs(om.H())
+    print(f"\n[fp32] dH/H = {dH:.3e}  dphiA = {np.abs(gm.get('phiA') - om.get('phiA')).max():.3e}", flush=True)
+    assert dH <= 1e-6

@@ -132,6 +132,17 @@ int po_model_relax(po_model *m, int max_iter, int min_iter, int record_interval,
 /* number of contour steps one update() advances (2 * Ns: qA, qB, qBc, qAc) */
 long po_model_contour_steps(const po_model *m);
 
+/* Measurement hooks; no counterpart in the reference (which records CPU
+ * clock() per iteration, src/scft.cc:398-412).  With profiling on, every
+ * update() records three HIP events on the model's stream: before the
+ * propagator chains, after them, and after the density / Anderson kernels.
+ * po_model_profile synchronises and returns out[0] = total propagator ms,
+ * out[1] = total density + Anderson ms, out[2] = updates profiled,
+ * out[3] = lockstep propagator steps per update (Ns; each advances qA|qB and
+ * qBc|qAc by one contour step).  Switching profiling on resets the totals. */
+int po_model_set_profiling(po_model *m, int on);
+int po_model_profile(po_model *m, double *out4);
+
 #ifdef __cplusplus
 }
 #endif

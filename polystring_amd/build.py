@@ -23,6 +23,11 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Wall", "-Wno-unused-command-line-argument"]
 SOURCES = ["propagator.hip", "kernels.hip", "capi.hip"]
+HOST = os.path.join(HERE, "host")
+HOST_SOURCES = ["mat5.cc", "Config.cc", "Propagator.cc", "Model_AB.cc", "scft.cc"]
+HOST_LIB = os.path.join(HERE, "libpolyorder_host.so")
+DRIVER = os.path.join(HERE, "polyorder_amd")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
 
 def _compile(src):
@@ -45,9 +50,30 @@ def build(verbose=True):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout + r.stderr)
     os.replace(tmp, LIB)
+    build_host()
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB}, {HOST_LIB}, {DRIVER}")
     return LIB
+
+
+def build_host():
+    """The C++ mirror of the reference's Config / Propagator / Model_AB / scft
+    classes (host-only, g++) and the command-line driver, linked against the
+    C ABI library."""
+    cxx = os.environ.get("CXX", "g++")
+    common = ["-O2", "-std=c++14", "-fPIC", "-Wall", f"-I{INCLUDE}", f"-I{HOST}"]
+    srcs = [os.path.join(HOST, f) for f in HOST_SOURCES]
+    run = lambda cmd: subprocess.run(cmd, capture_output=True, text=True)
+    r = run([cxx] + common + ["-shared", "-o", HOST_LIB + ".tmp"] + srcs +
+            [f"-L{HERE}", "-lpolystring_amd", "-lz", "-Wl,-rpath,$ORIGIN"])
+    if r.returncode != 0:
+        raise RuntimeError("host library build failed:\n" + r.stdout + r.stderr)
+    os.replace(HOST_LIB + ".tmp", HOST_LIB)
+    r = run([cxx] + common + ["-o", DRIVER + ".tmp", os.path.join(HOST, "polyorder_amd.cc"),
+                              f"-L{HERE}", "-lpolyorder_host", "-lpolystring_amd", "-Wl,-rpath,$ORIGIN"])
+    if r.returncode != 0:
+        raise RuntimeError("driver build failed:\n" + r.stdout + r.stderr)
+    os.replace(DRIVER + ".tmp", DRIVER)
 
 
 if __name__ == "__main__":

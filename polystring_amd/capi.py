@@ -31,7 +31,7 @@ EXPORTS = [
     "po_shape_supported", "po_prop_create", "po_prop_destroy", "po_prop_len", "po_prop_set_head",
     "po_prop_set_head_from_tail", "po_prop_solve", "po_prop_get_slice", "po_prop_Q", "po_density",
     "po_model_create", "po_model_destroy", "po_model_update", "po_model_observe", "po_model_get",
-    "po_model_relax", "po_model_contour_steps",
+    "po_model_relax", "po_model_contour_steps", "po_model_set_profiling", "po_model_profile",
 ]
 
 
@@ -82,6 +82,7 @@ def load():
         "po_model_get": ([_vp, i, _dp], i),
         "po_model_relax": ([_vp, i, i, i, d, ctypes.POINTER(i), _dp, _dp], i),
         "po_model_contour_steps": ([_vp], ctypes.c_long),
+        "po_model_set_profiling": ([_vp, i], i), "po_model_profile": ([_vp, _dp], i),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -235,3 +236,11 @@ class ModelAB:
 
     def contour_steps(self):
         return load().po_model_contour_steps(self.h)
+
+    def set_profiling(self, on=True):
+        _check(load().po_model_set_profiling(self.h, 1 if on else 0))
+
+    def profile(self):
+        o = np.zeros(4)
+        _check(load().po_model_profile(self.h, _p(o)))
+        return dict(prop_ms=o[0], rest_ms=o[1], iters=int(o[2]), lockstep_steps=int(o[3]))

@@ -372,14 +372,48 @@ template <typename T> struct Program {
 
     int run(const Shape &sh, const Roots<T> &rt, hipStream_t st) const {
         for (const auto &l : seq) {
-            hipError_t e = l.plane ? launch_plane<T>(sh, rt, pd.p + l.off, l.n, st)
+            hipError_t e = l.plane ? launch_plane<T>(sh, rt, pd.p + l.off, l.n, st, plane_variant)
                                    : launch_xpass<T>(sh, rt, xd.p + l.off, l.n, st);
             if (e != hipSuccess) return fail(PO_ERR_DEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
         }
         return PO_OK;
     }
 
+    // Autotuner: times the whole program once per plane-kernel variant (the
+    // workgroup size of the yz-plane kernel) on the live buffers and keeps
+    // the fastest.  Every slice and spectrum the program writes is rewritten
+    // by the next real run, so tuning leaves no trace in the results.
+    int tune(const Shape &sh, const Roots<T> &rt, hipStream_t st) {
+        const int nvar = (long)sh.Ly * sh.Lz >= 8192 ? 2 : 1;
+        if (nvar == 1) return PO_OK;
+        hipEvent_t a, b;
+        PO_HIP(hipEventCreate(&a));
+        PO_HIP(hipEventCreate(&b));
+        float best = 1e30f;
+        int pick = 0;
+        for (int v = 0; v < nvar; ++v) {
+            plane_variant = v;
+            int rc = run(sh, rt, st);  // warm-up
+            if (rc) return rc;
+            PO_HIP(hipEventRecord(a, st));
+            for (int rep = 0; rep < 2 && rc == PO_OK; ++rep) rc = run(sh, rt, st);
+            if (rc) return rc;
+            PO_HIP(hipEventRecord(b, st));
+            PO_HIP(hipEventSynchronize(b));
+            float ms = 0.f;
+            PO_HIP(hipEventElapsedTime(&ms, a, b));
+            tuned_ms[v] = ms / 2;
+            if (ms < best) { best = ms; pick = v; }
+        }
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        plane_variant = pick;
+        return PO_OK;
+    }
+
     long slice_elems = 0;  // M, set before build()
+    int plane_variant = 0;
+    float tuned_ms[2] = {0.f, 0.f};
 };
 
 }  // namespace po
@@ -488,7 +522,25 @@ struct po_model {
     po_ctx *ctx = nullptr;
     po_model_desc desc{};
     long steps = 0;
-    virtual ~po_model() = default;
+    // profiling: three events per update() -- before the propagator chains,
+    // after them, and after the density / Anderson kernels
+    bool profiling = false;
+    std::vector<hipEvent_t> ev;
+    size_t ev_used = 0;
+    int prof_iters = 0;
+    virtual ~po_model() {
+        for (auto e : ev) (void)hipEventDestroy(e);
+    }
+    hipError_t mark() {
+        if (!profiling) return hipSuccess;
+        if (ev_used == ev.size()) {
+            hipEvent_t e;
+            hipError_t r = hipEventCreate(&e);
+            if (r != hipSuccess) return r;
+            ev.push_back(e);
+        }
+        return hipEventRecord(ev[ev_used++], ctx->stream);
+    }
     virtual int update(int n) = 0;
     virtual int observe(double *out) = 0;
     virtual int get(int which, double *out) = 0;
@@ -601,7 +653,8 @@ template <typename T> struct Model : po_model {
         lanes[1].Sf = S[2].p;
         lanes[1].Sh = S[3].p;
         prog.slice_elems = M;
-        return prog.build(d.algo, lanes, st);
+        if ((rc = prog.build(d.algo, lanes, st))) return rc;
+        return prog.tune(sh, rt, st);
     }
 
     int find_index(int j) const {  // src/FieldAX.cc:262-267
@@ -614,8 +667,10 @@ template <typename T> struct Model : po_model {
         hipStream_t st = ctx->stream;
         const long M = sh.M;
         for (int it = 0; it < iters; ++it) {
+            PO_HIP(mark());
             int rc = prog.run(sh, rt, st);
             if (rc) return rc;
+            PO_HIP(mark());
             // Q = qB->Qt() (src/Model_AB.cc:195)
             PO_HIP(launch_mean_partials<T>(qB.p + (size_t)(sB - 1) * M, M, part.p, st));
             PO_HIP(launch_finish_partials(part.p, kRedBlocks, 1, (double)M, obs.p + OBS_Q, st));
@@ -641,6 +696,8 @@ template <typename T> struct Model : po_model {
             FactorOut<T> fo{eA2.p, eA4.p, eB2.p, eB4.p, -0.5 * dsA, -0.25 * dsA, -0.5 * dsB, -0.25 * dsB};
             PO_HIP(launch_anderson_mix<T>(a, C.p, wA.p, wB.p, phiA.p, phiB.p, fo, part.p, st));
             PO_HIP(launch_observables(part.p, M, obs.p, st));
+            PO_HIP(mark());
+            if (profiling) ++prof_iters;
             cur = a.next;
             ++k;
         }
@@ -880,5 +937,32 @@ int po_model_relax(po_model *m, int max_iter, int min_iter, int record_interval,
 }
 
 long po_model_contour_steps(const po_model *m) { return m ? m->steps : -1; }
+
+int po_model_set_profiling(po_model *m, int on) {
+    if (!m) return fail(PO_ERR_INVALID, "null model");
+    PO_HIP(hipStreamSynchronize(m->ctx->stream));
+    m->profiling = on != 0;
+    m->ev_used = 0;
+    m->prof_iters = 0;
+    return PO_OK;
+}
+
+int po_model_profile(po_model *m, double *out4) {
+    if (!m || !out4) return fail(PO_ERR_INVALID, "null argument");
+    PO_HIP(hipStreamSynchronize(m->ctx->stream));
+    double prop = 0.0, rest = 0.0;
+    for (int i = 0; i < m->prof_iters; ++i) {
+        float a = 0.f, b = 0.f;
+        PO_HIP(hipEventElapsedTime(&a, m->ev[3 * i], m->ev[3 * i + 1]));
+        PO_HIP(hipEventElapsedTime(&b, m->ev[3 * i + 1], m->ev[3 * i + 2]));
+        prop += a;
+        rest += b;
+    }
+    out4[0] = prop;
+    out4[1] = rest;
+    out4[2] = m->prof_iters;
+    out4[3] = (double)(m->steps / 2);  // lockstep launches of the two chains per update
+    return PO_OK;
+}
 
 }  // extern "C"

@@ -56,9 +56,10 @@ template <typename T> struct Roots {
 bool plane_supported(int Ly, int Lz);
 bool xpass_supported(int Lx);
 
+// variant 0: default workgroup size; 1: 1024 threads (planes of >= 8192 reals)
 template <typename T>
 hipError_t launch_plane(const Shape &sh, const Roots<T> &rt, const PlaneDesc<T> *d_desc, int nlanes,
-                        hipStream_t st);
+                        hipStream_t st, int variant = 0);
 template <typename T>
 hipError_t launch_xpass(const Shape &sh, const Roots<T> &rt, const XDesc<T> *d_desc, int nlanes,
                         hipStream_t st);

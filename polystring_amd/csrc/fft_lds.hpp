@@ -173,4 +173,191 @@ __device__ __forceinline__ void fft_lds(cx<T> *buf, int ps, int es, const cx<T> 
     if constexpr (N > 1) fft_stages<T, N, 0, 1, INV, NT, NP, PFAST>(buf, ps, es, roots);
 }
 
+// ---------------------------------------------------------- wave-owned --
+// Wave-level ordering point: every LDS access of this wave issued before it
+// completes before any issued after it (LDS instructions of one wave execute
+// in order; the fences keep the compiler from moving accesses across).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One Stockham stage over the GP pencils one wavefront owns (pencil g at
+// buf + g * ps).  Lanes loop over (pencil, butterfly) pairs; only this wave
+// touches these pencils, so stages need no workgroup barrier.
+template <typename T, int N, int R, int L, bool INV, int GP, bool PFAST>
+__device__ __forceinline__ void wave_stage(cx<T> *buf, int ps, int es, const cx<T> *__restrict__ roots, int lane) {
+    constexpr int NB = N / R;
+    constexpr int TOT = GP * NB;
+    constexpr int IT = (TOT + 63) / 64;
+    constexpr int TS = N / (L * R);
+    cx<T> v[IT][R];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int b = lane + it * 64;
+        if ((TOT % 64 == 0) || b < TOT) {
+            const int p = PFAST ? b % GP : b / NB;
+            const int j = PFAST ? b / GP : b % NB;
+            const cx<T> *src = buf + p * ps;
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[it][r] = src[(j + r * NB) * es];
+            if (L > 1) {
+                const int k = j % L;
+#pragma unroll
+                for (int r = 1; r < R; ++r) v[it][r] = cmul<INV, T>(v[it][r], roots[r * k * TS]);
+            }
+            Dft<R, INV, T>::run(v[it]);
+        }
+    }
+    wave_sync();
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int b = lane + it * 64;
+        if ((TOT % 64 == 0) || b < TOT) {
+            const int p = PFAST ? b % GP : b / NB;
+            const int j = PFAST ? b / GP : b % NB;
+            const int k = j % L;
+            cx<T> *dst = buf + p * ps + ((j / L) * L * R + k) * es;
+#pragma unroll
+            for (int r = 0; r < R; ++r) dst[r * L * es] = v[it][r];
+        }
+    }
+    wave_sync();
+}
+
+template <typename T, int N, int S, int L, bool INV, int GP, bool PFAST>
+__device__ __forceinline__ void wave_stages(cx<T> *buf, int ps, int es, const cx<T> *roots, int lane) {
+    if constexpr (S < Plan<N>::n) {
+        constexpr int R = Plan<N>::r[S];
+        wave_stage<T, N, R, L, INV, GP, PFAST>(buf, ps, es, roots, lane);
+        wave_stages<T, N, S + 1, L * R, INV, GP, PFAST>(buf, ps, es, roots, lane);
+    }
+}
+
+// In-place length-N transform of the GP pencils owned by the calling wave.
+template <typename T, int N, bool INV, int GP, bool PFAST>
+__device__ __forceinline__ void fft_wave(cx<T> *buf, int ps, int es, const cx<T> *roots, int lane) {
+    if constexpr (N > 1) wave_stages<T, N, 0, 1, INV, GP, PFAST>(buf, ps, es, roots, lane);
+}
+
+// ------------------------------------------------- register hand-off --
+// Stage inputs and outputs can come from / go to LDS, registers or global
+// memory.  With Stockham ordering the first stage (L = 1) reads positions
+// j + r N/R of a pencil and the last stage (L = N/R) writes positions
+// j + r N/R, so when an inverse transform ends with radix R and the next
+// forward transform starts with the same radix, the elements a lane holds
+// after the one are exactly those it needs for the other: the data stays in
+// registers.  Inverse transforms run the forward radix plan reversed, so the
+// radix at each end of a forward/inverse pair matches.
+enum { IO_LDS = 0, IO_REG = 1, IO_GLB = 2 };
+
+template <int N, bool INV, int S> struct StageRadix {
+    static constexpr int value = INV ? Plan<N>::r[Plan<N>::n - 1 - S] : Plan<N>::r[S];
+};
+
+// radix and per-lane butterfly count at the first / last stage of a wave transform
+template <int N, bool INV, int GP> struct WaveEnds {
+    static constexpr int RF = StageRadix<N, INV, 0>::value;
+    static constexpr int RL = StageRadix<N, INV, Plan<N>::n - 1>::value;
+    static constexpr int ITF = (GP * (N / RF) + 63) / 64;
+    static constexpr int ITL = (GP * (N / RL) + 63) / 64;
+};
+
+template <typename T, int N, int R, int L, bool INV, int GP, bool PFAST, int IN, int OUT, int IT>
+__device__ __forceinline__ void wave_stage_io(cx<T> *lds, int ps, int es, const cx<T> *__restrict__ roots, int lane,
+                                              const cx<T> *gin, long gps, long ges, cx<T> *gout, cx<T> (&v)[IT][R]) {
+    constexpr int NB = N / R;
+    constexpr int TOT = GP * NB;
+    constexpr int TS = N / (L * R);
+    static_assert(IT == (TOT + 63) / 64, "register array does not match the stage");
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int b = lane + it * 64;
+        if ((TOT % 64 == 0) || b < TOT) {
+            const int p = PFAST ? b % GP : b / NB;
+            const int j = PFAST ? b / GP : b % NB;
+            if constexpr (IN == IO_LDS) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) v[it][r] = lds[p * ps + (j + r * NB) * es];
+            } else if constexpr (IN == IO_GLB) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) v[it][r] = gin[p * gps + (long)(j + r * NB) * ges];
+            }
+            if (L > 1) {
+                const int k = j % L;
+#pragma unroll
+                for (int r = 1; r < R; ++r) v[it][r] = cmul<INV, T>(v[it][r], roots[r * k * TS]);
+            }
+            Dft<R, INV, T>::run(v[it]);
+        }
+    }
+    if constexpr (OUT == IO_LDS) {
+        if constexpr (IN == IO_LDS) wave_sync();
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const int b = lane + it * 64;
+            if ((TOT % 64 == 0) || b < TOT) {
+                const int p = PFAST ? b % GP : b / NB;
+                const int j = PFAST ? b / GP : b % NB;
+                const int k = j % L;
+                cx<T> *dst = lds + p * ps + ((j / L) * L * R + k) * es;
+#pragma unroll
+                for (int r = 0; r < R; ++r) dst[r * L * es] = v[it][r];
+            }
+        }
+        wave_sync();
+    } else if constexpr (OUT == IO_GLB) {
+        static_assert(L * R == N, "global output only from the last stage");
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const int b = lane + it * 64;
+            if ((TOT % 64 == 0) || b < TOT) {
+                const int p = PFAST ? b % GP : b / NB;
+                const int j = PFAST ? b / GP : b % NB;
+#pragma unroll
+                for (int r = 0; r < R; ++r) gout[p * gps + (long)(j + r * L) * ges] = v[it][r];
+            }
+        }
+    }
+}
+
+// A whole transform of the GP pencils a wave owns, first stage reading from
+// IN, last stage writing to OUT (intermediate stages go through LDS).
+template <typename T, int N, bool INV, int GP, bool PFAST, int IN, int OUT, int S = 0, int L = 1>
+struct WaveFFT {
+    static constexpr int NS = Plan<N>::n;
+    static constexpr int R = StageRadix<N, INV, S>::value;
+    static constexpr int IT = (GP * (N / R) + 63) / 64;
+    using E = WaveEnds<N, INV, GP>;
+    using Next = WaveFFT<T, N, INV, GP, PFAST, IN, OUT, S + 1, L * R>;
+    static __device__ __forceinline__ void run(cx<T> *lds, int ps, int es, const cx<T> *roots, int lane,
+                                               const cx<T> *gin, long gps, long ges, cx<T> *gout,
+                                               cx<T> (&vin)[E::ITF][E::RF], cx<T> (&vout)[E::ITL][E::RL]) {
+        constexpr bool first = S == 0, last = S == NS - 1;
+        constexpr int in_mode = first ? IN : IO_LDS;
+        constexpr int out_mode = last ? OUT : IO_LDS;
+        if constexpr (first && IN == IO_REG) {
+            static_assert(!(last && OUT == IO_REG) || NS == 1, "");
+            wave_stage_io<T, N, R, L, INV, GP, PFAST, in_mode, out_mode, IT>(lds, ps, es, roots, lane, gin, gps, ges,
+                                                                            gout, vin);
+            if constexpr (last && OUT == IO_REG) {
+#pragma unroll
+                for (int i = 0; i < IT; ++i)
+#pragma unroll
+                    for (int r = 0; r < R; ++r) vout[i][r] = vin[i][r];
+            }
+            if constexpr (!last) Next::run(lds, ps, es, roots, lane, gin, gps, ges, gout, vin, vout);
+        } else if constexpr (last && OUT == IO_REG) {
+            wave_stage_io<T, N, R, L, INV, GP, PFAST, in_mode, out_mode, IT>(lds, ps, es, roots, lane, gin, gps, ges,
+                                                                            gout, vout);
+        } else {
+            cx<T> v[IT][R];
+            wave_stage_io<T, N, R, L, INV, GP, PFAST, in_mode, out_mode, IT>(lds, ps, es, roots, lane, gin, gps, ges,
+                                                                            gout, v);
+            if constexpr (!last) Next::run(lds, ps, es, roots, lane, gin, gps, ges, gout, vin, vout);
+        }
+    }
+};
+
 }  // namespace po

@@ -21,44 +21,45 @@
 
 namespace po {
 
-template <typename T, int LY, int LZ> struct PlaneCfg {
+template <typename T, int LY, int LZ, int NTV = 0> struct PlaneCfg {
     static constexpr int H = LZ / 2;
     static constexpr int HZ = LZ == 1 ? 1 : H + 1;
     static constexpr int NE = LY * LZ;  // reals per plane
     static constexpr int NC = LY * HZ;  // complex per spectrum plane
-    // largest of 512/256/128/64 threads giving every thread >= 16 reals;
-    // 512 leaves a 256-VGPR budget for the staged butterflies.
-    static constexpr int NT = NE >= 16 * 512 ? 512 : NE >= 16 * 256 ? 256 : NE >= 16 * 128 ? 128 : 64;
+    // default: largest of 512/256/128/64 threads giving every thread >= 16
+    // reals (512 leaves a 256-VGPR budget for the staged butterflies)
+    static constexpr int NT = NTV ? NTV : NE >= 16 * 512 ? 512 : NE >= 16 * 256 ? 256 : NE >= 16 * 128 ? 128 : 64;
     static constexpr int EPT = (NE + NT - 1) / NT;
+    static constexpr int NW = NT / 64;
+    // rows (z pencils) and regular columns (y pencils) owned per wave
+    static constexpr int RPW = LZ > 1 ? LY / NW : 0;
+    static constexpr int CPW = LZ > 1 ? H / NW : 0;
+    // LDS row stride in complex: consecutive rows start CPW complex apart
+    // modulo the 64 four-byte banks, so a wave reading CPW adjacent columns
+    // down 64 / CPW rows touches every bank equally often
+    static constexpr int CW = sizeof(cx<T>) / 4;
+    static constexpr int BANKMOD = 64 / CW;
+    static constexpr int RS = LZ == 1 ? 1 : HZ + ((CPW - HZ % BANKMOD) % BANKMOD + BANKMOD) % BANKMOD;
+    static constexpr int NL = LY * RS;  // LDS complex per plane
+    static_assert(LZ == 1 || (LY % NW == 0 && H % NW == 0), "plane shape must split evenly over waves");
 };
 
-// y transform of the HZ columns of a plane.  With LZ > 1 the H = LZ/2
-// regular columns form one batch and the Nyquist column (kz = H) a second
-// one, so the main batch maps evenly onto the threads and needs fewer staged
-// butterflies (registers) per thread.
-template <typename T, int LY, int LZ, bool INV, int NT>
-__device__ __forceinline__ void fft_columns(cx<T> *buf, const Roots<T> &rt) {
-    constexpr int H = LZ / 2;
-    constexpr int HZ = LZ == 1 ? 1 : H + 1;
-    if constexpr (LZ == 1) {
-        fft_lds<T, LY, INV, NT, 1, true>(buf, 1, 1, rt.y);
-    } else {
-        fft_lds<T, LY, INV, NT, H, true>(buf, 1, HZ, rt.y);
-        fft_lds<T, LY, INV, NT, 1, true>(buf + H, 1, HZ, rt.y);
-    }
-}
-
-// real-to-complex along z (rows), then forward y (columns)
+// real-to-complex along z (rows, each wave its own RPW rows), then forward
+// y (columns, each wave its own CPW columns; the last wave also takes the
+// Nyquist column kz = H).  Ends with a workgroup barrier.
 template <typename T, int LY, int LZ, int NT>
 __device__ __forceinline__ void plane_forward(cx<T> *buf, const Roots<T> &rt) {
-    constexpr int H = LZ / 2;
-    constexpr int HZ = LZ == 1 ? 1 : H + 1;
+    using C = PlaneCfg<T, LY, LZ, NT>;
+    constexpr int H = C::H, RS = C::RS, NW = C::NW;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     if constexpr (LZ > 1) {
-        fft_lds<T, H, false, NT, LY, false>(buf, HZ, 1, rt.zh);
+        constexpr int RPW = C::RPW;
+        cx<T> *rows = buf + w * RPW * RS;
+        fft_wave<T, H, false, RPW, false>(rows, RS, 1, rt.zh, lane);
         constexpr int NPAIR = H / 2 + 1;
-        for (int t = threadIdx.x; t < LY * NPAIR; t += NT) {
+        for (int t = lane; t < RPW * NPAIR; t += 64) {
             const int y = t / NPAIR, k = t % NPAIR;
-            cx<T> *row = buf + y * HZ;
+            cx<T> *row = rows + y * RS;
             if (k == 0) {
                 const cx<T> z0 = row[0];
                 row[0] = mk<T>(z0.x + z0.y, (T)0);
@@ -72,28 +73,40 @@ __device__ __forceinline__ void plane_forward(cx<T> *buf, const Roots<T> &rt) {
                 const cx<T> oa = rot<false, T>(scl<T>(sub<T>(a, conj<T>(b)), hf));
                 const cx<T> eb = scl<T>(add<T>(b, conj<T>(a)), hf);
                 const cx<T> ob = rot<false, T>(scl<T>(sub<T>(b, conj<T>(a)), hf));
-                const cx<T> xa = add<T>(ea, cmul<false, T>(oa, rt.z[k]));
-                const cx<T> xb = add<T>(eb, cmul<false, T>(ob, rt.z[k2]));
-                row[k] = xa;
-                row[k2] = xb;
+                row[k] = add<T>(ea, cmul<false, T>(oa, rt.z[k]));
+                row[k2] = add<T>(eb, cmul<false, T>(ob, rt.z[k2]));
             }
         }
         __syncthreads();
+        if constexpr (LY > 1) {
+            fft_wave<T, LY, false, C::CPW, true>(buf + w * C::CPW, 1, RS, rt.y, lane);
+            if (w == NW - 1) fft_wave<T, LY, false, 1, true>(buf + H, 1, RS, rt.y, lane);
+        }
+    } else if constexpr (LY > 1) {
+        if (w == 0) fft_wave<T, LY, false, 1, true>(buf, 1, 1, rt.y, lane);
     }
-    if constexpr (LY > 1) fft_columns<T, LY, LZ, false, NT>(buf, rt);
+    __syncthreads();
 }
 
-// inverse y (columns), then complex-to-real along z (rows); unnormalised
+// inverse y (columns), then complex-to-real along z (rows); unnormalised.
+// Ends with a workgroup barrier.
 template <typename T, int LY, int LZ, int NT>
 __device__ __forceinline__ void plane_inverse(cx<T> *buf, const Roots<T> &rt) {
-    constexpr int H = LZ / 2;
-    constexpr int HZ = LZ == 1 ? 1 : H + 1;
-    if constexpr (LY > 1) fft_columns<T, LY, LZ, true, NT>(buf, rt);
+    using C = PlaneCfg<T, LY, LZ, NT>;
+    constexpr int H = C::H, RS = C::RS, NW = C::NW;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     if constexpr (LZ > 1) {
+        if constexpr (LY > 1) {
+            fft_wave<T, LY, true, C::CPW, true>(buf + w * C::CPW, 1, RS, rt.y, lane);
+            if (w == NW - 1) fft_wave<T, LY, true, 1, true>(buf + H, 1, RS, rt.y, lane);
+            __syncthreads();
+        }
+        constexpr int RPW = C::RPW;
+        cx<T> *rows = buf + w * RPW * RS;
         constexpr int NPAIR = H / 2 + 1;
-        for (int t = threadIdx.x; t < LY * NPAIR; t += NT) {
+        for (int t = lane; t < RPW * NPAIR; t += 64) {
             const int y = t / NPAIR, k = t % NPAIR;
-            cx<T> *row = buf + y * HZ;
+            cx<T> *row = rows + y * RS;
             if (k == 0) {
                 const cx<T> x0 = row[0], xh = row[H];
                 const cx<T> e = add<T>(x0, conj<T>(xh));
@@ -110,28 +123,34 @@ __device__ __forceinline__ void plane_inverse(cx<T> *buf, const Roots<T> &rt) {
                 row[k2] = add<T>(eb, rot<true, T>(ob));
             }
         }
-        __syncthreads();
-        fft_lds<T, H, true, NT, LY, false>(buf, HZ, 1, rt.zh);
+        wave_sync();
+        fft_wave<T, H, true, RPW, false>(rows, RS, 1, rt.zh, lane);
+    } else if constexpr (LY > 1) {
+        if (w == 0) fft_wave<T, LY, true, 1, true>(buf, 1, 1, rt.y, lane);
     }
+    __syncthreads();
 }
 
-template <typename T, int LY, int LZ>
-__global__ void __launch_bounds__((PlaneCfg<T, LY, LZ>::NT))
+template <typename T, int LY, int LZ, int NTV>
+__global__ void __launch_bounds__((PlaneCfg<T, LY, LZ, NTV>::NT))
 k_plane(const PlaneDesc<T> *__restrict__ descs, Roots<T> rt, T invM) {
-    using C = PlaneCfg<T, LY, LZ>;
-    constexpr int NT = C::NT, NE = C::NE, NC = C::NC, EPT = C::EPT, HZ = C::HZ;
-    __shared__ cx<T> buf[NC];
+    using C = PlaneCfg<T, LY, LZ, NTV>;
+    constexpr int NT = C::NT, NE = C::NE, NC = C::NC, EPT = C::EPT, HZ = C::HZ, RS = C::RS;
+    __shared__ cx<T> buf[C::NL];
     T *rbuf = reinterpret_cast<T *>(buf);
     const PlaneDesc<T> d = descs[blockIdx.y];
     const long ro = (long)blockIdx.x * NE;
     const long co = (long)blockIdx.x * NC;
     const int tid = threadIdx.x;
-    auto lpos = [](int e) { return (e / LZ) * 2 * HZ + (e % LZ); };
+    // real element e = (y, z) of the plane <-> LDS real index; spectrum
+    // element c = (y, kz) <-> LDS complex index
+    auto lpos = [](int e) { return (e / LZ) * 2 * RS + (e % LZ); };
+    auto cpos = [](int c) { return (c / HZ) * RS + (c % HZ); };
     T q[EPT];
 
     if (d.in0) {
 #pragma unroll 8
-        for (int c = tid; c < NC; c += NT) buf[c] = d.in0[co + c];
+        for (int c = tid; c < NC; c += NT) buf[cpos(c)] = d.in0[co + c];
         __syncthreads();
         plane_inverse<T, LY, LZ, NT>(buf, rt);
 #pragma unroll
@@ -145,7 +164,7 @@ k_plane(const PlaneDesc<T> *__restrict__ descs, Roots<T> rt, T invM) {
         if (d.in1) {
             __syncthreads();
 #pragma unroll 8
-            for (int c = tid; c < NC; c += NT) buf[c] = d.in1[co + c];
+            for (int c = tid; c < NC; c += NT) buf[cpos(c)] = d.in1[co + c];
             __syncthreads();
             plane_inverse<T, LY, LZ, NT>(buf, rt);
 #pragma unroll
@@ -190,7 +209,7 @@ k_plane(const PlaneDesc<T> *__restrict__ descs, Roots<T> rt, T invM) {
         __syncthreads();
         plane_forward<T, LY, LZ, NT>(buf, rt);
 #pragma unroll 8
-        for (int c = tid; c < NC; c += NT) d.out0[co + c] = buf[c];
+        for (int c = tid; c < NC; c += NT) d.out0[co + c] = buf[cpos(c)];
     }
     if (d.out1) {
         // The slice is re-read from where this thread stored it (qout, or the
@@ -209,7 +228,211 @@ k_plane(const PlaneDesc<T> *__restrict__ descs, Roots<T> rt, T invM) {
         __syncthreads();
         plane_forward<T, LY, LZ, NT>(buf, rt);
 #pragma unroll 8
-        for (int c = tid; c < NC; c += NT) d.out1[co + c] = buf[c];
+        for (int c = tid; c < NC; c += NT) d.out1[co + c] = buf[cpos(c)];
+    }
+}
+
+// R2C split on the RPW rows a wave owns (after the forward z transform).
+template <typename T, int H, int RPW, int RS>
+__device__ __forceinline__ void rows_r2c_split(cx<T> *rows, const cx<T> *__restrict__ rz, int lane) {
+    constexpr int NPAIR = H / 2 + 1;
+    for (int t = lane; t < RPW * NPAIR; t += 64) {
+        const int y = t / NPAIR, k = t % NPAIR;
+        cx<T> *row = rows + y * RS;
+        if (k == 0) {
+            const cx<T> z0 = row[0];
+            row[0] = mk<T>(z0.x + z0.y, (T)0);
+            row[H] = mk<T>(z0.x - z0.y, (T)0);
+        } else {
+            const int k2 = H - k;
+            const cx<T> a = row[k], b = row[k2];
+            const T hf = (T)0.5;
+            const cx<T> ea = scl<T>(add<T>(a, conj<T>(b)), hf);
+            const cx<T> oa = rot<false, T>(scl<T>(sub<T>(a, conj<T>(b)), hf));
+            const cx<T> eb = scl<T>(add<T>(b, conj<T>(a)), hf);
+            const cx<T> ob = rot<false, T>(scl<T>(sub<T>(b, conj<T>(a)), hf));
+            row[k] = add<T>(ea, cmul<false, T>(oa, rz[k]));
+            row[k2] = add<T>(eb, cmul<false, T>(ob, rz[k2]));
+        }
+    }
+}
+
+// C2R pre-split on the RPW rows a wave owns (before the inverse z transform).
+template <typename T, int H, int RPW, int RS>
+__device__ __forceinline__ void rows_c2r_split(cx<T> *rows, const cx<T> *__restrict__ rz, int lane) {
+    constexpr int NPAIR = H / 2 + 1;
+    for (int t = lane; t < RPW * NPAIR; t += 64) {
+        const int y = t / NPAIR, k = t % NPAIR;
+        cx<T> *row = rows + y * RS;
+        if (k == 0) {
+            const cx<T> x0 = row[0], xh = row[H];
+            const cx<T> e = add<T>(x0, conj<T>(xh));
+            const cx<T> o = sub<T>(x0, conj<T>(xh));
+            row[0] = add<T>(e, rot<true, T>(o));
+        } else {
+            const int k2 = H - k;
+            const cx<T> a = row[k], b = row[k2];
+            const cx<T> ea = add<T>(a, conj<T>(b));
+            const cx<T> oa = cmul<true, T>(sub<T>(a, conj<T>(b)), rz[k]);
+            const cx<T> eb = add<T>(b, conj<T>(a));
+            const cx<T> ob = cmul<true, T>(sub<T>(b, conj<T>(a)), rz[k2]);
+            row[k] = add<T>(ea, rot<true, T>(oa));
+            row[k2] = add<T>(eb, rot<true, T>(ob));
+        }
+    }
+}
+
+// 3-D plane kernel with register hand-off.  Same contract as k_plane.  The
+// slice lives in registers in "row layout": lane element [it][r] is the
+// complex pair (q[p][2m], q[p][2m+1]) of row p = w*RPW + b/NBZ at pair index
+// m = b%NBZ + r*NBZ, b = lane + 64*it -- exactly what the last stage of the
+// inverse z transform produces and the first stage of the forward one takes.
+// The y transforms read their input spectrum from and write their output
+// spectrum to HBM directly in their first / last stage.
+template <typename T, int LY, int LZ, int NTV>
+__global__ void __launch_bounds__((PlaneCfg<T, LY, LZ, NTV>::NT))
+k_plane3(const PlaneDesc<T> *__restrict__ descs, Roots<T> rt, T invM) {
+    using C = PlaneCfg<T, LY, LZ, NTV>;
+    constexpr int H = C::H, HZ = C::HZ, RS = C::RS, NW = C::NW, RPW = C::RPW, CPW = C::CPW;
+    constexpr int NE = C::NE, NC = C::NC;
+    using ZF = WaveEnds<H, false, RPW>;
+    using ZI = WaveEnds<H, true, RPW>;
+    static_assert(ZI::RL == ZF::RF && ZI::ITL == ZF::ITF, "z transform ends must match");
+    constexpr int RZ = ZF::RF, ITZ = ZF::ITF, NBZ = H / RZ, TOTZ = RPW * NBZ;
+    using YI = WaveEnds<LY, true, CPW>;
+    using YF = WaveEnds<LY, false, CPW>;
+    using YI1 = WaveEnds<LY, true, 1>;
+    using YF1 = WaveEnds<LY, false, 1>;
+    __shared__ cx<T> buf[C::NL];
+    const PlaneDesc<T> d = descs[blockIdx.y];
+    const long ro = (long)blockIdx.x * NE;
+    const long co = (long)blockIdx.x * NC;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    cx<T> *rows = buf + w * RPW * RS;
+
+    // pair pointer of row-layout element (it, r) inside a real plane array
+    auto pair = [&](const T *base, int it, int r) -> const cx<T> * {
+        const int b = lane + it * 64;
+        const int p = w * RPW + b / NBZ, m = b % NBZ + r * NBZ;
+        return reinterpret_cast<const cx<T> *>(base + ro + (long)p * LZ) + m;
+    };
+    auto valid = [&](int it) { return (TOTZ % 64 == 0) || (lane + it * 64) < TOTZ; };
+
+    auto inverse_to_regs = [&](const cx<T> *src, cx<T> (&v)[ITZ][RZ]) {
+        __syncthreads();  // LDS free
+        {
+            cx<T> a[YI::ITF][YI::RF], b[YI::ITL][YI::RL];
+            WaveFFT<T, LY, true, CPW, true, IO_GLB, IO_LDS>::run(buf + w * CPW, 1, RS, rt.y, lane, src + co + w * CPW,
+                                                                 1, HZ, nullptr, a, b);
+        }
+        if (w == NW - 1) {
+            cx<T> a[YI1::ITF][YI1::RF], b[YI1::ITL][YI1::RL];
+            WaveFFT<T, LY, true, 1, true, IO_GLB, IO_LDS>::run(buf + H, 1, RS, rt.y, lane, src + co + H, 1, HZ,
+                                                               nullptr, a, b);
+        }
+        __syncthreads();
+        rows_c2r_split<T, H, RPW, RS>(rows, rt.z, lane);
+        wave_sync();
+        {
+            cx<T> a[ZI::ITF][ZI::RF];
+            WaveFFT<T, H, true, RPW, false, IO_LDS, IO_REG>::run(rows, RS, 1, rt.zh, lane, nullptr, 0, 0, nullptr, a,
+                                                                 v);
+        }
+    };
+    auto forward_from_regs = [&](cx<T> (&v)[ITZ][RZ], cx<T> *dst) {
+        __syncthreads();  // LDS free
+        {
+            cx<T> b[ZF::ITL][ZF::RL];
+            WaveFFT<T, H, false, RPW, false, IO_REG, IO_LDS>::run(rows, RS, 1, rt.zh, lane, nullptr, 0, 0, nullptr, v,
+                                                                  b);
+        }
+        rows_r2c_split<T, H, RPW, RS>(rows, rt.z, lane);
+        __syncthreads();
+        {
+            cx<T> a[YF::ITF][YF::RF], b[YF::ITL][YF::RL];
+            WaveFFT<T, LY, false, CPW, true, IO_LDS, IO_GLB>::run(buf + w * CPW, 1, RS, rt.y, lane, nullptr, 1, HZ,
+                                                                  dst + co + w * CPW, a, b);
+        }
+        if (w == NW - 1) {
+            cx<T> a[YF1::ITF][YF1::RF], b[YF1::ITL][YF1::RL];
+            WaveFFT<T, LY, false, 1, true, IO_LDS, IO_GLB>::run(buf + H, 1, RS, rt.y, lane, nullptr, 1, HZ,
+                                                                dst + co + H, a, b);
+        }
+    };
+
+    cx<T> q[ITZ][RZ];
+    if (d.in0) {
+        inverse_to_regs(d.in0, q);
+#pragma unroll
+        for (int it = 0; it < ITZ; ++it)
+            if (valid(it)) {
+#pragma unroll
+                for (int r = 0; r < RZ; ++r) {
+                    const cx<T> f = d.post0 ? *pair(d.post0, it, r) : mk<T>((T)1, (T)1);
+                    q[it][r] = mk<T>((q[it][r].x * f.x) * invM, (q[it][r].y * f.y) * invM);
+                }
+            }
+        if (d.in1) {
+            cx<T> h[ITZ][RZ];
+            inverse_to_regs(d.in1, h);
+#pragma unroll
+            for (int it = 0; it < ITZ; ++it)
+                if (valid(it)) {
+#pragma unroll
+                    for (int r = 0; r < RZ; ++r) {
+                        const cx<T> f = d.post1 ? *pair(d.post1, it, r) : mk<T>((T)1, (T)1);
+                        const T hx = (h[it][r].x * f.x) * invM, hy = (h[it][r].y * f.y) * invM;
+                        // src/RQM4.cc:91
+                        q[it][r] = mk<T>(((T)4 * hx - q[it][r].x) / (T)3, ((T)4 * hy - q[it][r].y) / (T)3);
+                    }
+                }
+        }
+    } else {
+#pragma unroll
+        for (int it = 0; it < ITZ; ++it)
+            if (valid(it)) {
+#pragma unroll
+                for (int r = 0; r < RZ; ++r) q[it][r] = *pair(d.qin, it, r);
+            }
+    }
+    if (d.qout) {
+#pragma unroll
+        for (int it = 0; it < ITZ; ++it)
+            if (valid(it)) {
+#pragma unroll
+                for (int r = 0; r < RZ; ++r) {
+                    *const_cast<cx<T> *>(pair(d.qout, it, r)) = q[it][r];
+                    if (d.qout2) *const_cast<cx<T> *>(pair(d.qout2, it, r)) = q[it][r];
+                }
+            }
+    }
+    if (d.out0) {
+#pragma unroll
+        for (int it = 0; it < ITZ; ++it)
+            if (valid(it)) {
+#pragma unroll
+                for (int r = 0; r < RZ; ++r) {
+                    const cx<T> f = d.pre0 ? *pair(d.pre0, it, r) : mk<T>((T)1, (T)1);
+                    q[it][r] = mk<T>(f.x * q[it][r].x, f.y * q[it][r].y);
+                }
+            }
+        forward_from_regs(q, d.out0);
+    }
+    if (d.out1) {
+        // the slice is re-read from where this lane stored it (qout, or the
+        // input slice) rather than kept in registers across out0
+        const T *qs = d.qout ? d.qout : d.qin;
+#pragma unroll
+        for (int it = 0; it < ITZ; ++it)
+            if (valid(it)) {
+#pragma unroll
+                for (int r = 0; r < RZ; ++r) {
+                    const cx<T> f = d.pre1 ? *pair(d.pre1, it, r) : mk<T>((T)1, (T)1);
+                    const cx<T> v = *pair(qs, it, r);
+                    q[it][r] = mk<T>(f.x * v.x, f.y * v.y);
+                }
+            }
+        forward_from_regs(q, d.out1);
     }
 }
 
@@ -270,13 +493,25 @@ bool xpass_supported(int Lx) {
 
 template <typename T>
 hipError_t launch_plane(const Shape &sh, const Roots<T> &rt, const PlaneDesc<T> *d_desc, int nlanes,
-                        hipStream_t st) {
+                        hipStream_t st, int variant) {
     const T invM = (T)(1.0 / (double)sh.M);
-#define PO_CASE(a, b)                                                                           \
-    if (sh.Ly == a && sh.Lz == b) {                                                             \
-        hipLaunchKernelGGL((k_plane<T, a, b>), dim3(sh.Lx, nlanes), dim3(PlaneCfg<T, a, b>::NT), \
-                           0, st, d_desc, rt, invM);                                            \
-        return hipGetLastError();                                                               \
+#define PO_CASE(a, b)                                                                                 \
+    if (sh.Ly == a && sh.Lz == b) {                                                                   \
+        if constexpr (a * b >= 8192) {                                                                \
+            if (variant == 1) {                                                                       \
+                hipLaunchKernelGGL((k_plane3<T, a, b, 1024>), dim3(sh.Lx, nlanes), dim3(1024), 0, st, d_desc, rt, \
+                                   invM);                                                             \
+                return hipGetLastError();                                                             \
+            }                                                                                         \
+        }                                                                                             \
+        if constexpr (b > 1 && a > 1) {                                                               \
+            hipLaunchKernelGGL((k_plane3<T, a, b, 0>), dim3(sh.Lx, nlanes), dim3(PlaneCfg<T, a, b, 0>::NT), 0, st, \
+                               d_desc, rt, invM);                                                     \
+        } else {                                                                                      \
+            hipLaunchKernelGGL((k_plane<T, a, b, 0>), dim3(sh.Lx, nlanes), dim3(PlaneCfg<T, a, b, 0>::NT), 0, st, \
+                               d_desc, rt, invM);                                                     \
+        }                                                                                             \
+        return hipGetLastError();                                                                     \
     }
     PO_PLANE_SHAPES(PO_CASE)
 #undef PO_CASE
@@ -300,9 +535,9 @@ hipError_t launch_xpass(const Shape &sh, const Roots<T> &rt, const XDesc<T> *d_d
 }
 
 template hipError_t launch_plane<double>(const Shape &, const Roots<double> &, const PlaneDesc<double> *, int,
-                                         hipStream_t);
+                                         hipStream_t, int);
 template hipError_t launch_plane<float>(const Shape &, const Roots<float> &, const PlaneDesc<float> *, int,
-                                        hipStream_t);
+                                        hipStream_t, int);
 template hipError_t launch_xpass<double>(const Shape &, const Roots<double> &, const XDesc<double> *, int,
                                          hipStream_t);
 template hipError_t launch_xpass<float>(const Shape &, const Roots<float> &, const XDesc<float> *, int,

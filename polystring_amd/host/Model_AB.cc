@@ -1,0 +1,196 @@
+// Model_AB.cc -- host side of the AB model: configuration, File init of the
+// fields from a MAT v5 file, observables and MAT output.
+#include "Model_AB.h"
+
+#include <cmath>
+#include <cstdlib>
+#include <iomanip>
+#include <iostream>
+#include <map>
+
+#include "Propagator.h"
+#include "mat5.h"
+
+Model_AB::Model_AB(const std::string config_file) : Model(config_file) { init(); }
+
+Model_AB::Model_AB(const Config &cfg) : Model(cfg) { init(); }
+
+Model_AB::~Model_AB() {
+    if (_m) po_model_destroy(_m);
+}
+
+void Model_AB::init() {
+    if (_cfg.model() != ModelType::AB) {
+        std::cerr << "Model_AB: [Model] model must be AB" << std::endl;
+        std::exit(1);
+    }
+    if (_cfg.algo_scft_type() != AlgorithmSCFTType::ANDERSON || _cfg.is_compressible()) {
+        std::cerr << "Model_AB: this build runs the incompressible Anderson-mixing path only" << std::endl;
+        std::exit(1);
+    }
+    if (_cfg.algo_mde_type() == AlgorithmMDEType::ETDRK4) {
+        std::cerr << "Model_AB: [Algorithm_MDE] algorithm must be RQM4 or OS2" << std::endl;
+        std::exit(1);
+    }
+    _uc = UnitCell(_cfg);
+    _Lx = (int)_cfg.Lx();
+    _Ly = (int)_cfg.Ly();
+    _Lz = (int)_cfg.Lz();
+    _chiN = _cfg.chiN().at(0);
+    const std::vector<unsigned> Ms = _cfg.Ms();
+    _sA = Ms[0];
+    _sB = Ms[1];
+    // src/Model_AB.cc:100-111
+    const unsigned LA = _sA - 1, LB = _sB - 1;
+    _fA = 1.0 * LA / (LA + LB);
+    _fB = 1.0 * LB / (LA + LB);
+    _dsA = _fA / LA;
+    _dsB = _fB / LB;
+    // File init: MAT variables wA, wB (src/Model_AB.cc:633-644, src/Grid.cc:694-711).
+    // A missing file is an error here; the reference silently fell back to
+    // time-seeded random fields.
+    std::map<std::string, mat5::Var> vars;
+    std::string err;
+    const std::string file = _cfg.field_data_file();
+    if (!mat5::read(file, vars, err) || !vars.count("wA") || !vars.count("wB")) {
+        std::cerr << "Model_AB: cannot read wA, wB from field_data file '" << file << "': " << err << std::endl;
+        std::exit(1);
+    }
+    const size_t M = (size_t)_Lx * _Ly * _Lz;
+    if (vars["wA"].data.size() != M || vars["wB"].data.size() != M) {
+        std::cerr << "Model_AB: field_data size does not match Lx*Ly*Lz" << std::endl;
+        std::exit(1);
+    }
+    std::vector<double> wA = mat5::fortran_to_c(vars["wA"].data.data(), _Lx, _Ly, _Lz);
+    std::vector<double> wB = mat5::fortran_to_c(vars["wB"].data.data(), _Lx, _Ly, _Lz);
+    const std::vector<double> lam = _cfg.lam();
+    po_model_desc d;
+    d.grid = _uc.grid_desc(_Lx, _Ly, _Lz);
+    d.sA = (int)_sA;
+    d.sB = (int)_sB;
+    d.chiN = _chiN;
+    d.lamA = lam.size() > 0 ? lam[0] : 0.1;
+    d.lamB = lam.size() > 1 ? lam[1] : 0.1;
+    d.n_anderson = (int)_cfg.n_Anderson_mixing();
+    d.algo = _cfg.algo_mde_type() == AlgorithmMDEType::OS2 ? PO_OS2 : PO_RQM4;
+    d.contour = _cfg.algo_contour_integration_type() == AlgorithmContourType::TRAPEZOIDAL ? PO_QUAD_TRAP : PO_QUAD_AUTO;
+    po_host_check(po_model_create(po_host_context(&_cfg), &d, wA.data(), wB.data(), &_m), "po_model_create");
+}
+
+void Model_AB::update() {
+    po_host_check(po_model_update(_m, 1), "po_model_update");
+    _obs_valid = false;
+}
+
+void Model_AB::observe() const {
+    if (_obs_valid) return;
+    po_host_check(po_model_observe(_m, _obs), "po_model_observe");
+    _obs_valid = true;
+}
+
+double Model_AB::H() const { observe(); return _obs[0]; }
+double Model_AB::Hw() const { observe(); return _obs[1]; }
+double Model_AB::Hs() const { observe(); return _obs[2]; }
+double Model_AB::residual_error() const { observe(); return _obs[3]; }
+double Model_AB::Q() const { observe(); return _obs[4]; }
+
+std::vector<double> Model_AB::field(int which) const {
+    std::vector<double> out((size_t)_Lx * _Ly * _Lz);
+    po_host_check(po_model_get(_m, which, out.data()), "po_model_get");
+    return out;
+}
+
+double Model_AB::incomp() const {
+    const std::vector<double> a = field(PO_FIELD_PHIA), b = field(PO_FIELD_PHIB);
+    double s = 0.0;
+    for (size_t i = 0; i < a.size(); ++i) s += std::fabs(a[i] + b[i] - 1.0);
+    return s / (double)a.size();
+}
+
+static void stats(const std::vector<double> &v, double &mean, double &mn, double &mx) {
+    double s = 0.0;
+    mn = v[0];
+    mx = v[0];
+    for (double x : v) {
+        s += x;
+        mn = x < mn ? x : mn;
+        mx = x > mx ? x : mx;
+    }
+    mean = s / (double)v.size();
+}
+
+void Model_AB::display() const {
+    std::cout << "\tUnit Cell: " << _uc.type() << std::endl;
+    std::cout.setf(std::ios::fixed, std::ios::floatfield);
+    std::cout.precision(4);
+    std::cout << "\t(lx,ly,lz) = " << _uc.lx() << "," << _uc.ly() << "," << _uc.lz() << std::endl;
+    std::cout.precision(6);
+    std::cout << "\tH    = " << H() << " = " << Hw() << " + " << Hs() << std::endl;
+    std::cout.precision(4);
+    const char *names[4] = {"wA  ", "wB  ", "phiA", "phiB"};
+    for (int f : {PO_FIELD_PHIA, PO_FIELD_PHIB, PO_FIELD_WA, PO_FIELD_WB}) {
+        double mean, mn, mx;
+        stats(field(f), mean, mn, mx);
+        std::cout << "\t" << names[f] << " = " << mean << "\t[" << mn << ", " << mx << "]" << std::endl;
+    }
+    std::cout.setf(std::ios::scientific, std::ios::floatfield);
+    std::cout.precision(2);
+    std::cout << "\tQ_qB = " << Q() << std::endl;
+    std::cout << "\tResidual Error    = " << residual_error() << std::endl;
+    std::cout.unsetf(std::ios::floatfield);
+    std::cout.precision(6);
+}
+
+void Model_AB::display_parameters() const {
+    std::cout << "\tAB diblock melt: fA = " << _fA << ", chiN = " << _chiN << ", Ms = [" << _sA << ", " << _sB
+              << "], ds = " << _dsA << std::endl;
+    std::cout << "\tGrid " << _Lx << " x " << _Ly << " x " << _Lz << ", cell " << _uc.type() << std::endl;
+}
+
+void Model_AB::save(const std::string file) {
+    save_field(file);
+    save_density(file);
+}
+
+// Variables are written in Fortran order like CMatFile (include/CMatFile.h).
+void Model_AB::save_field(const std::string file) {
+    mat5::Writer w(file);
+    if (!w.ok()) {
+        std::cerr << "error: cannot open MAT-file: " << file << std::endl;
+        return;
+    }
+    const std::vector<int> dims = {_Lx, _Ly, _Lz};
+    std::vector<double> a = field(PO_FIELD_WA), b = field(PO_FIELD_WB), pa = field(PO_FIELD_PHIA),
+                        pb = field(PO_FIELD_PHIB);
+    std::vector<double> yita(a.size());
+    for (size_t i = 0; i < a.size(); ++i) yita[i] = 0.5 * ((a[i] + b[i]) - _chiN);
+    w.put("wA", dims, mat5::c_to_fortran(a.data(), _Lx, _Ly, _Lz).data());
+    w.put("wB", dims, mat5::c_to_fortran(b.data(), _Lx, _Ly, _Lz).data());
+    w.put("yita", dims, mat5::c_to_fortran(yita.data(), _Lx, _Ly, _Lz).data());
+    w.put("phiA", dims, mat5::c_to_fortran(pa.data(), _Lx, _Ly, _Lz).data());
+    w.put("phiB", dims, mat5::c_to_fortran(pb.data(), _Lx, _Ly, _Lz).data());
+}
+
+void Model_AB::save_density(const std::string file) { (void)file; /* written with the fields */ }
+
+void Model_AB::save_model(const std::string file) {
+    mat5::Writer w(file);
+    if (!w.ok()) return;
+    w.put_scalar("fA", _fA);
+    w.put_scalar("fB", _fB);
+    w.put_scalar("chiN", _chiN);
+    w.put_scalar("dsA", _dsA);
+    w.put_scalar("dsB", _dsB);
+    w.put_scalar("sA", _sA);
+    w.put_scalar("sB", _sB);
+    w.put_scalar("dim", _uc.dim());
+    w.put_scalar("Lx", _Lx);
+    w.put_scalar("Ly", _Ly);
+    w.put_scalar("Lz", _Lz);
+    w.put_scalar("lx", _uc.lx());
+    w.put_scalar("ly", _uc.ly());
+    w.put_scalar("lz", _uc.lz());
+    w.put_string("crystal_system_type", _uc.type());
+}
+
+void Model_AB::save_q(const std::string file) { (void)file; /* slices stay on the GPU */ }

@@ -91,14 +91,30 @@ def cpu_baseline(seconds_target=15.0):
                        f"complex-to-complex FFTs as the reference), {dt:.1f} s on {O.num_threads()} threads")
 
 
+def copy_probe(nbytes=1 << 30, reps=3):
+    """Plain device-to-device copy of a known size (torch), the calibration
+    kernel for rocprofv3 FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    x = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": nbytes, "copy_probe_bytes_written": nbytes, "reps": reps}))
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--copy-probe", action="store_true", help="run only the traffic-calibration copy")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (each GPU would run "
               "independent solves / string images); use --gpus 1", file=sys.stderr)

@@ -20,7 +20,8 @@ namespace po {
 // Per-lane work descriptor of the fused yz-plane kernel.  One workgroup
 // handles one x-plane of one lane:
 //   q   = in0 ? (inv(in0) * post0 / M) : qin
-//   q   = in1 ? (4 * inv(in1) * post1 / M - q) / 3 : q    (RQM4 combine)
+//   q   = in1 ? (4 * inv(in1) * post1 / M - q) / 3 : q    (RQM4 combine;
+//         needs qout, which holds the full-step slice in between)
 //   qout, qout2 <- q                                        (optional stores)
 //   outj = fwd(q * prej)  for j = 0, 1                      (optional)
 // inv() is the inverse y transform followed by complex-to-real along z;

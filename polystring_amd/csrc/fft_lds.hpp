@@ -23,6 +23,35 @@ template <> struct vec2<float> { using type = float2; };
 template <typename T> using cx = typename vec2<T>::type;
 
 template <typename T> __device__ __forceinline__ cx<T> mk(T a, T b) { cx<T> r; r.x = a; r.y = b; return r; }
+
+// Loads / stores through the global address space: pointers read from a
+// descriptor table are generic to the compiler, which would otherwise emit
+// flat instructions for them.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define PO_GLOBAL __attribute__((address_space(1)))
+#else
+#define PO_GLOBAL
+#endif
+__device__ __forceinline__ double gload(const double *p) { return *(const PO_GLOBAL double *)p; }
+__device__ __forceinline__ float gload(const float *p) { return *(const PO_GLOBAL float *)p; }
+__device__ __forceinline__ double2 gload(const double2 *p) {
+    const PO_GLOBAL double *q = (const PO_GLOBAL double *)p;
+    return mk<double>(q[0], q[1]);
+}
+__device__ __forceinline__ float2 gload(const float2 *p) {
+    const PO_GLOBAL float *q = (const PO_GLOBAL float *)p;
+    return mk<float>(q[0], q[1]);
+}
+__device__ __forceinline__ void gstore(double2 *p, double2 v) {
+    PO_GLOBAL double *q = (PO_GLOBAL double *)p;
+    q[0] = v.x;
+    q[1] = v.y;
+}
+__device__ __forceinline__ void gstore(float2 *p, float2 v) {
+    PO_GLOBAL float *q = (PO_GLOBAL float *)p;
+    q[0] = v.x;
+    q[1] = v.y;
+}
 template <typename T> __device__ __forceinline__ cx<T> add(cx<T> a, cx<T> b) { return mk<T>(a.x + b.x, a.y + b.y); }
 template <typename T> __device__ __forceinline__ cx<T> sub(cx<T> a, cx<T> b) { return mk<T>(a.x - b.x, a.y - b.y); }
 template <typename T> __device__ __forceinline__ cx<T> scl(cx<T> a, T s) { return mk<T>(a.x * s, a.y * s); }
@@ -266,7 +295,7 @@ template <int N, bool INV, int GP> struct WaveEnds {
 
 template <typename T, int N, int R, int L, bool INV, int GP, bool PFAST, int IN, int OUT, int IT>
 __device__ __forceinline__ void wave_stage_io(cx<T> *lds, int ps, int es, const cx<T> *__restrict__ roots, int lane,
-                                              const cx<T> *gin, long gps, long ges, cx<T> *gout, cx<T> (&v)[IT][R]) {
+                                              const cx<T> *gin, int gps, int ges, cx<T> *gout, cx<T> (&v)[IT][R]) {
     constexpr int NB = N / R;
     constexpr int TOT = GP * NB;
     constexpr int TS = N / (L * R);
@@ -282,7 +311,7 @@ __device__ __forceinline__ void wave_stage_io(cx<T> *lds, int ps, int es, const 
                 for (int r = 0; r < R; ++r) v[it][r] = lds[p * ps + (j + r * NB) * es];
             } else if constexpr (IN == IO_GLB) {
 #pragma unroll
-                for (int r = 0; r < R; ++r) v[it][r] = gin[p * gps + (long)(j + r * NB) * ges];
+                for (int r = 0; r < R; ++r) v[it][r] = gload(gin + (unsigned)(p * gps + (j + r * NB) * ges));
             }
             if (L > 1) {
                 const int k = j % L;
@@ -316,7 +345,7 @@ __device__ __forceinline__ void wave_stage_io(cx<T> *lds, int ps, int es, const 
                 const int p = PFAST ? b % GP : b / NB;
                 const int j = PFAST ? b / GP : b % NB;
 #pragma unroll
-                for (int r = 0; r < R; ++r) gout[p * gps + (long)(j + r * L) * ges] = v[it][r];
+                for (int r = 0; r < R; ++r) gstore(gout + (unsigned)(p * gps + (j + r * L) * ges), v[it][r]);
             }
         }
     }
@@ -332,7 +361,7 @@ struct WaveFFT {
     using E = WaveEnds<N, INV, GP>;
     using Next = WaveFFT<T, N, INV, GP, PFAST, IN, OUT, S + 1, L * R>;
     static __device__ __forceinline__ void run(cx<T> *lds, int ps, int es, const cx<T> *roots, int lane,
-                                               const cx<T> *gin, long gps, long ges, cx<T> *gout,
+                                               const cx<T> *gin, int gps, int ges, cx<T> *gout,
                                                cx<T> (&vin)[E::ITF][E::RF], cx<T> (&vout)[E::ITL][E::RL]) {
         constexpr bool first = S == 0, last = S == NS - 1;
         constexpr int in_mode = first ? IN : IO_LDS;

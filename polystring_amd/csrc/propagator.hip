@@ -307,14 +307,15 @@ k_plane3(const PlaneDesc<T> *__restrict__ descs, Roots<T> rt, T invM) {
     const PlaneDesc<T> d = descs[blockIdx.y];
     const long ro = (long)blockIdx.x * NE;
     const long co = (long)blockIdx.x * NC;
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     cx<T> *rows = buf + w * RPW * RS;
 
-    // pair pointer of row-layout element (it, r) inside a real plane array
+    // pair pointer of row-layout element (it, r) inside a real plane array:
+    // a block-uniform base plus a 32-bit offset, so addresses stay one VGPR
     auto pair = [&](const T *base, int it, int r) -> const cx<T> * {
         const int b = lane + it * 64;
-        const int p = w * RPW + b / NBZ, m = b % NBZ + r * NBZ;
-        return reinterpret_cast<const cx<T> *>(base + ro + (long)p * LZ) + m;
+        const unsigned off = (unsigned)((w * RPW + b / NBZ) * (LZ / 2) + b % NBZ + r * NBZ);
+        return reinterpret_cast<const cx<T> *>(base + ro) + off;
     };
     auto valid = [&](int it) { return (TOTZ % 64 == 0) || (lane + it * 64) < TOTZ; };
 
@@ -368,22 +369,31 @@ k_plane3(const PlaneDesc<T> *__restrict__ descs, Roots<T> rt, T invM) {
             if (valid(it)) {
 #pragma unroll
                 for (int r = 0; r < RZ; ++r) {
-                    const cx<T> f = d.post0 ? *pair(d.post0, it, r) : mk<T>((T)1, (T)1);
+                    const cx<T> f = d.post0 ? gload(pair(d.post0, it, r)) : mk<T>((T)1, (T)1);
                     q[it][r] = mk<T>((q[it][r].x * f.x) * invM, (q[it][r].y * f.y) * invM);
                 }
             }
         if (d.in1) {
-            cx<T> h[ITZ][RZ];
-            inverse_to_regs(d.in1, h);
+            // The full-step slice q_f waits in the output slice (the same
+            // lane reads back what it wrote; the lines are still in L2)
+            // instead of in registers across the second inverse transform.
+#pragma unroll
+            for (int it = 0; it < ITZ; ++it)
+                if (valid(it)) {
+#pragma unroll
+                    for (int r = 0; r < RZ; ++r) gstore(const_cast<cx<T> *>(pair(d.qout, it, r)), q[it][r]);
+                }
+            inverse_to_regs(d.in1, q);
 #pragma unroll
             for (int it = 0; it < ITZ; ++it)
                 if (valid(it)) {
 #pragma unroll
                     for (int r = 0; r < RZ; ++r) {
-                        const cx<T> f = d.post1 ? *pair(d.post1, it, r) : mk<T>((T)1, (T)1);
-                        const T hx = (h[it][r].x * f.x) * invM, hy = (h[it][r].y * f.y) * invM;
+                        const cx<T> f = d.post1 ? gload(pair(d.post1, it, r)) : mk<T>((T)1, (T)1);
+                        const T hx = (q[it][r].x * f.x) * invM, hy = (q[it][r].y * f.y) * invM;
+                        const cx<T> qf = gload(pair(d.qout, it, r));
                         // src/RQM4.cc:91
-                        q[it][r] = mk<T>(((T)4 * hx - q[it][r].x) / (T)3, ((T)4 * hy - q[it][r].y) / (T)3);
+                        q[it][r] = mk<T>(((T)4 * hx - qf.x) / (T)3, ((T)4 * hy - qf.y) / (T)3);
                     }
                 }
         }
@@ -392,7 +402,7 @@ k_plane3(const PlaneDesc<T> *__restrict__ descs, Roots<T> rt, T invM) {
         for (int it = 0; it < ITZ; ++it)
             if (valid(it)) {
 #pragma unroll
-                for (int r = 0; r < RZ; ++r) q[it][r] = *pair(d.qin, it, r);
+                for (int r = 0; r < RZ; ++r) q[it][r] = gload(pair(d.qin, it, r));
             }
     }
     if (d.qout) {
@@ -401,8 +411,8 @@ k_plane3(const PlaneDesc<T> *__restrict__ descs, Roots<T> rt, T invM) {
             if (valid(it)) {
 #pragma unroll
                 for (int r = 0; r < RZ; ++r) {
-                    *const_cast<cx<T> *>(pair(d.qout, it, r)) = q[it][r];
-                    if (d.qout2) *const_cast<cx<T> *>(pair(d.qout2, it, r)) = q[it][r];
+                    gstore(const_cast<cx<T> *>(pair(d.qout, it, r)), q[it][r]);
+                    if (d.qout2) gstore(const_cast<cx<T> *>(pair(d.qout2, it, r)), q[it][r]);
                 }
             }
     }
@@ -412,7 +422,7 @@ k_plane3(const PlaneDesc<T> *__restrict__ descs, Roots<T> rt, T invM) {
             if (valid(it)) {
 #pragma unroll
                 for (int r = 0; r < RZ; ++r) {
-                    const cx<T> f = d.pre0 ? *pair(d.pre0, it, r) : mk<T>((T)1, (T)1);
+                    const cx<T> f = d.pre0 ? gload(pair(d.pre0, it, r)) : mk<T>((T)1, (T)1);
                     q[it][r] = mk<T>(f.x * q[it][r].x, f.y * q[it][r].y);
                 }
             }
@@ -427,8 +437,8 @@ k_plane3(const PlaneDesc<T> *__restrict__ descs, Roots<T> rt, T invM) {
             if (valid(it)) {
 #pragma unroll
                 for (int r = 0; r < RZ; ++r) {
-                    const cx<T> f = d.pre1 ? *pair(d.pre1, it, r) : mk<T>((T)1, (T)1);
-                    const cx<T> v = *pair(qs, it, r);
+                    const cx<T> f = d.pre1 ? gload(pair(d.pre1, it, r)) : mk<T>((T)1, (T)1);
+                    const cx<T> v = gload(pair(qs, it, r));
                     q[it][r] = mk<T>(f.x * v.x, f.y * v.y);
                 }
             }
@@ -468,6 +478,65 @@ k_xpass(const XDesc<T> *__restrict__ descs, Roots<T> rt, int ncol, int hz) {
     for (int idx = tid; idx < LX * C; idx += NT) {
         const int x = idx / C, col = c0 + idx % C;
         if (col < ncol) d.S[(long)x * ncol + col] = buf[idx];
+    }
+}
+
+// x-pencil kernel with register hand-off: each wave owns CPW adjacent
+// (y,kz) columns of the tile; the forward x transform reads them from HBM in
+// its first stage and ends in registers, where K is applied; the inverse
+// starts from those registers and writes HBM in its last stage.  Only the
+// middle stages use LDS.  Needs ncol to be a multiple of C (full tiles).
+template <typename T, int LX> struct X3Cfg {
+    static constexpr int C = 16;
+    // LX*C/8 threads (one radix-8 butterfly each) rounded up to a power of two
+    static constexpr int NT0 = LX * C / 8;
+    static constexpr int NT = NT0 <= 64 ? 64 : NT0 <= 128 ? 128 : NT0 <= 256 ? 256 : 512;
+    static constexpr int NW = NT / 64;
+    static constexpr int CPW = C / NW;
+    static constexpr int CW = sizeof(cx<T>) / 4;
+    static constexpr int BANKMOD = 64 / CW;
+    static constexpr int RS = C + ((CPW - C % BANKMOD) % BANKMOD + BANKMOD) % BANKMOD;
+    static_assert(C % NW == 0, "tile columns must split evenly over waves");
+};
+
+template <typename T, int LX>
+__global__ void __launch_bounds__((X3Cfg<T, LX>::NT))
+k_xpass3(const XDesc<T> *__restrict__ descs, Roots<T> rt, int ncol, int hz) {
+    using X = X3Cfg<T, LX>;
+    constexpr int CPW = X::CPW, RS = X::RS;
+    using F = WaveEnds<LX, false, CPW>;
+    using I = WaveEnds<LX, true, CPW>;
+    static_assert(F::RL == I::RF && F::ITL == I::ITF, "x transform ends must match");
+    constexpr int RX = F::RL, ITX = F::ITL, NBX = LX / RX, TOTX = CPW * NBX;
+    __shared__ cx<T> buf[LX * RS];
+    const XDesc<T> d = descs[blockIdx.y];
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int cw = blockIdx.x * X::C + w * CPW;  // first column of this wave
+    cx<T> *lds = buf + w * CPW;
+    cx<T> v[ITX][RX];
+    {
+        cx<T> a[F::ITF][F::RF];
+        WaveFFT<T, LX, false, CPW, true, IO_GLB, IO_REG>::run(lds, 1, RS, rt.x, lane, d.S + cw, 1, ncol, nullptr, a,
+                                                              v);
+    }
+#pragma unroll
+    for (int it = 0; it < ITX; ++it) {
+        const int b = lane + it * 64;
+        if ((TOTX % 64 == 0) || b < TOTX) {
+            const int c = cw + b % CPW, j = b / CPW;
+#pragma unroll
+            for (int r = 0; r < RX; ++r) {
+                const int x = j + r * NBX;
+                const T f = d.tab ? gload(d.tab + (unsigned)(x * ncol + c))
+                                  : gload(d.fx + x) * gload(d.fy + c / hz) * gload(d.fz + c % hz);
+                v[it][r] = scl<T>(v[it][r], f);
+            }
+        }
+    }
+    {
+        cx<T> b[I::ITL][I::RL];
+        WaveFFT<T, LX, true, CPW, true, IO_REG, IO_GLB>::run(lds, 1, RS, rt.x, lane, nullptr, 1, ncol, d.S + cw, v,
+                                                             b);
     }
 }
 
@@ -522,12 +591,17 @@ template <typename T>
 hipError_t launch_xpass(const Shape &sh, const Roots<T> &rt, const XDesc<T> *d_desc, int nlanes,
                         hipStream_t st) {
     const int ncol = sh.Ly * sh.Hz;
-#define PO_CASE(a)                                                                         \
-    if (sh.Lx == a) {                                                                      \
-        const int nt = (ncol + XCfg<T, a>::C - 1) / XCfg<T, a>::C;                         \
-        hipLaunchKernelGGL((k_xpass<T, a>), dim3(nt, nlanes), dim3(XCfg<T, a>::NT), 0, st, \
-                           d_desc, rt, ncol, sh.Hz);                                       \
-        return hipGetLastError();                                                          \
+#define PO_CASE(a)                                                                              \
+    if (sh.Lx == a) {                                                                           \
+        if (ncol % X3Cfg<T, a>::C == 0) {                                                       \
+            hipLaunchKernelGGL((k_xpass3<T, a>), dim3(ncol / X3Cfg<T, a>::C, nlanes), dim3(X3Cfg<T, a>::NT), 0, st, \
+                               d_desc, rt, ncol, sh.Hz);                                        \
+        } else {                                                                                \
+            const int nt = (ncol + XCfg<T, a>::C - 1) / XCfg<T, a>::C;                          \
+            hipLaunchKernelGGL((k_xpass<T, a>), dim3(nt, nlanes), dim3(XCfg<T, a>::NT), 0, st, d_desc, rt, ncol, \
+                               sh.Hz);                                                          \
+        }                                                                                       \
+        return hipGetLastError();                                                               \
     }
     PO_X_SIZES(PO_CASE)
 #undef PO_CASE

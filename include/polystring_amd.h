@@ -132,6 +132,35 @@ int po_model_relax(po_model *m, int max_iter, int min_iter, int record_interval,
 /* number of contour steps one update() advances (2 * Ns: qA, qB, qBc, qAc) */
 long po_model_contour_steps(const po_model *m);
 
+/* ------------------------------------------------------ adjoint / cell -- */
+/* Cell stress by the adjoint of the discrete propagator chain.  Replaces the
+ * reference's brute-force cell search, which re-solves the SCFT at each trial
+ * size (scft::bracket / optimization_Brent, src/scft.cc:124-309).  Runs the
+ * four propagators with the current fields, then the adjoint pass over the
+ * stored slices, and returns dH/dl_d for the cell lengths l = (a, b, c) along
+ * x, y, z at fixed fields: H depends on the cell only through -ln Q, so
+ * dH/dl_d = -(1/Q) dQ/dl_d (src/Model_AB.cc:263-315).  Entries d >= dim are 0;
+ * a cubic cell's dH/da is the sum of the three.  Q (optional) gets Q.  Needs
+ * separable k tables: RQM4 on any cell (it uses the orthogonal k^2,
+ * src/RQM4.cc:9-16), OS2 on orthogonal cells. */
+int po_model_stress(po_model *m, double *dHdl3, double *Q);
+/* New cell lengths and angles (radians; same crystal system and grid), as a
+ * reference cell-optimisation step re-initialises UnitCell and the updaters
+ * (src/scft.cc:361-380).  The fields are kept and the Anderson history
+ * restarts from them, as Model_AB::reset re-initialises it
+ * (src/Model_AB.cc:20-28). */
+int po_model_set_cell(po_model *m, double a, double b, double c, double alpha, double beta, double gamma);
+/* Stress-driven cell relaxation, the gradient replacement of the reference's
+ * Brent search over cell sizes (scft::run with AlgorithmCellType::BRENT,
+ * src/scft.cc:40-105,217-309).  Each outer step relaxes the fields as
+ * po_model_relax does, takes po_model_stress, and moves the crystal system's
+ * free lengths (a; a, c; a, b; a, b, c) by a per-length secant step; angles
+ * stay fixed.  Stops when every free-length stress |dH/dl| < stress_tol, or
+ * after max_outer steps.  cell6 (optional) gets a, b, c, alpha, beta, gamma;
+ * stress3 the free-length stresses (zero-padded); *outer the steps run. */
+int po_model_relax_cell(po_model *m, int max_outer, double stress_tol, int max_iter, int min_iter,
+                        int record_interval, double thresh, double *cell6, double *H, double *stress3, int *outer);
+
 /* Measurement hooks; no counterpart in the reference (which records CPU
  * clock() per iteration, src/scft.cc:398-412).  With profiling on, every
  * update() records three HIP events on the model's stream: before the

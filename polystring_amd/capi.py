@@ -31,7 +31,9 @@ EXPORTS = [
     "po_shape_supported", "po_prop_create", "po_prop_destroy", "po_prop_len", "po_prop_set_head",
     "po_prop_set_head_from_tail", "po_prop_solve", "po_prop_get_slice", "po_prop_Q", "po_density",
     "po_model_create", "po_model_destroy", "po_model_update", "po_model_observe", "po_model_get",
-    "po_model_relax", "po_model_contour_steps", "po_model_set_profiling", "po_model_profile",
+    "po_model_relax", "po_model_contour_steps", "po_model_stress", "po_model_set_cell",
+    "po_model_relax_cell",
+    "po_model_set_profiling", "po_model_profile",
 ]
 
 
@@ -82,6 +84,8 @@ def load():
         "po_model_get": ([_vp, i, _dp], i),
         "po_model_relax": ([_vp, i, i, i, d, ctypes.POINTER(i), _dp, _dp], i),
         "po_model_contour_steps": ([_vp], ctypes.c_long),
+        "po_model_stress": ([_vp, _dp, _dp], i), "po_model_set_cell": ([_vp] + [d] * 6, i),
+        "po_model_relax_cell": ([_vp, i, d, i, i, i, d, _dp, _dp, _dp, ctypes.POINTER(i)], i),
         "po_model_set_profiling": ([_vp, i], i), "po_model_profile": ([_vp, _dp], i),
     }
     for name, (args, res) in sig.items():
@@ -236,6 +240,29 @@ class ModelAB:
 
     def contour_steps(self):
         return load().po_model_contour_steps(self.h)
+
+    def stress(self):
+        """(dH/dl over the cell lengths along x, y, z at the current fields, Q)."""
+        g = np.zeros(3)
+        q = np.zeros(1)
+        _check(load().po_model_stress(self.h, _p(g), _p(q)))
+        return g, float(q[0])
+
+    def relax_cell(self, max_outer=30, stress_tol=1e-6, max_iter=2000, min_iter=10, record_interval=1,
+                   thresh=1e-10):
+        """Stress-driven cell relaxation; returns (cell6, H, stress3, outer steps)."""
+        cell = np.zeros(6)
+        H = np.zeros(1)
+        st = np.zeros(3)
+        k = ctypes.c_int(0)
+        _check(load().po_model_relax_cell(self.h, int(max_outer), float(stress_tol), int(max_iter), int(min_iter),
+                                          int(record_interval), float(thresh), _p(cell), _p(H), _p(st),
+                                          ctypes.byref(k)))
+        return cell, float(H[0]), st, k.value
+
+    def set_cell(self, a, b=0.0, c=0.0, alpha=0.0, beta=0.0, gamma=0.0):
+        _check(load().po_model_set_cell(self.h, float(a), float(b), float(c), float(alpha), float(beta),
+                                        float(gamma)))
 
     def set_profiling(self, on=True):
         _check(load().po_model_set_profiling(self.h, 1 if on else 0))

@@ -5,6 +5,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <array>
+#include <algorithm>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -143,25 +145,42 @@ static inline int rmin(int i, int L) { return i < L / 2 + 1 ? i : L - i; }
 // exp(-h * min over 27 images |g (n + shift)|^2) (src/UnitCell.cc:321-359).
 template <typename T> struct KTable {
     DBuf<T> fx, fy, fz, tab;
-    XDesc<T> desc(cx<T> *S) const { return XDesc<T>{S, fx.p, fy.p, fz.p, tab.p}; }
+    // dK/dl_d = K g_d with g_d[r] = 2 h (2 pi r / l_d)^2 / l_d (separable tables only)
+    DBuf<T> gx, gy, gz;
+    XDesc<T> desc(cx<T> *S, cx<T> *Sout = nullptr) const { return XDesc<T>{S, fx.p, fy.p, fz.p, tab.p, Sout}; }
 };
+
+// (Re)allocate only on a size change, so descriptors that hold the pointer
+// stay valid when a table is refilled for a new cell.
+template <typename T> static hipError_t ensure(DBuf<T> &b, size_t n) { return b.n == n && b.p ? hipSuccess : b.alloc(n); }
+
+template <typename T> static int upload(DBuf<T> &b, const std::vector<T> &h, hipStream_t st) {
+    PO_HIP(ensure(b, h.size()));
+    PO_HIP(hipMemcpyAsync(b.p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, st));
+    return PO_OK;
+}
 
 template <typename T>
 static int build_ktable(KTable<T> &kt, const Shape &sh, const Cell &cell, bool general, double h, hipStream_t st) {
     const double pi = 3.14159265358979323846264338327950288;
     if (!general) {
-        double cc[3] = {0, 0, 0};
-        for (int d = 0; d < cell.dim; ++d) cc[d] = -4.0 * pi * pi / (cell.l[d] * cell.l[d]);
-        std::vector<T> fx(sh.Lx), fy(sh.Ly), fz(sh.Hz);
-        for (int i = 0; i < sh.Lx; ++i) { const int r = rmin(i, sh.Lx); fx[i] = (T)std::exp(h * (cc[0] * r * r)); }
-        for (int j = 0; j < sh.Ly; ++j) { const int r = rmin(j, sh.Ly); fy[j] = (T)std::exp(h * (cc[1] * r * r)); }
-        for (int k = 0; k < sh.Hz; ++k) { const int r = rmin(k, sh.Lz); fz[k] = (T)std::exp(h * (cc[2] * r * r)); }
-        PO_HIP(kt.fx.alloc(sh.Lx));
-        PO_HIP(kt.fy.alloc(sh.Ly));
-        PO_HIP(kt.fz.alloc(sh.Hz));
-        PO_HIP(hipMemcpyAsync(kt.fx.p, fx.data(), sizeof(T) * sh.Lx, hipMemcpyHostToDevice, st));
-        PO_HIP(hipMemcpyAsync(kt.fy.p, fy.data(), sizeof(T) * sh.Ly, hipMemcpyHostToDevice, st));
-        PO_HIP(hipMemcpyAsync(kt.fz.p, fz.data(), sizeof(T) * sh.Hz, hipMemcpyHostToDevice, st));
+        double cc[3] = {0, 0, 0}, gc[3] = {0, 0, 0};
+        for (int d = 0; d < cell.dim; ++d) {
+            cc[d] = -4.0 * pi * pi / (cell.l[d] * cell.l[d]);
+            gc[d] = -2.0 * h * cc[d] / cell.l[d];
+        }
+        const int N[3] = {sh.Lx, sh.Ly, sh.Hz}, P[3] = {sh.Lx, sh.Ly, sh.Lz};
+        DBuf<T> *F[3] = {&kt.fx, &kt.fy, &kt.fz}, *G[3] = {&kt.gx, &kt.gy, &kt.gz};
+        int rc;
+        for (int d = 0; d < 3; ++d) {
+            std::vector<T> f(N[d]), g(N[d]);
+            for (int i = 0; i < N[d]; ++i) {
+                const double r = rmin(i, P[d]);
+                f[i] = (T)std::exp(h * (cc[d] * r * r));
+                g[i] = (T)(gc[d] * r * r);
+            }
+            if ((rc = upload(*F[d], f, st)) || (rc = upload(*G[d], g, st))) return rc;
+        }
         PO_HIP(hipStreamSynchronize(st));
         return PO_OK;
     }
@@ -186,8 +205,8 @@ static int build_ktable(KTable<T> &kt, const Shape &sh, const Cell &cell, bool g
                         }
                 tab[((size_t)i * sh.Ly + j) * sh.Hz + k] = (T)std::exp(-h * q2min);
             }
-    PO_HIP(kt.tab.alloc(sh.Mh));
-    PO_HIP(hipMemcpyAsync(kt.tab.p, tab.data(), sizeof(T) * sh.Mh, hipMemcpyHostToDevice, st));
+    int rc;
+    if ((rc = upload(kt.tab, tab, st))) return rc;
     PO_HIP(hipStreamSynchronize(st));
     return PO_OK;
 }
@@ -577,6 +596,8 @@ struct po_model {
     virtual int update(int n) = 0;
     virtual int observe(double *out) = 0;
     virtual int get(int which, double *out) = 0;
+    virtual int stress(double *dHdl, double *Q) = 0;
+    virtual int set_cell(const double *lengths_angles) = 0;
 };
 
 namespace po {
@@ -746,6 +767,191 @@ template <typename T> struct Model : po_model {
         out[2] = o[OBS_HS];
         out[3] = o[OBS_RESIDUAL];
         out[4] = o[OBS_Q];
+        return PO_OK;
+    }
+
+    // ------------------------------------------------------- adjoint --
+    // dQ/dl_d through the discrete chain.  With q_s the forward slices (qA
+    // then qB) and qc the complementary ones, the steps are symmetric
+    // operators A_s, so
+    //   dQ/dl_d = (1/M) sum_s <qc_{N-1-s}, (dA_s/dl_d) q_s>,
+    // and for S = E F^H K F E / M,  <a, dS b> = (1/M) sum_k dK conj(F E a) F E b.
+    // RQM4, A = (4 S_h S_h - S_f) / 3, needs three spectrum pairs per step:
+    //   (F E4 v, F E4 S_h u), (F E4 S_h v, F E4 u) with dK_h, weight 4/3,
+    //   (F E2 v, F E2 u) with dK_f, weight -1/3;  OS2 (A = S_f) the last with weight 1.
+    // Steps are independent given the stored slices, so they run in batches
+    // of adj_nb steps, two lanes (u, v) per step, per launch.
+    DBuf<cx<T>> adjS;
+    DBuf<PlaneDesc<T>> adjP1, adjP2;
+    DBuf<XDesc<T>> adjX;
+    DBuf<DotDesc<T>> adjD;
+    DBuf<double> adjPart, adjSum;
+    int adj_nb = 0, adj_N = 0;
+    bool adj_ready = false;
+
+    int adj_build() {
+        if (adj_ready) return PO_OK;
+        const bool rqm4 = desc.algo == PO_RQM4;
+        if (!rqm4 && !cell.ortho)
+            return fail(PO_ERR_INVALID, "stress needs separable k tables (RQM4, or OS2 on an orthogonal cell)");
+        hipStream_t st = ctx->stream;
+        const int nA = sA - 1, nB = sB - 1, N = nA + nB;
+        const int nsp = rqm4 ? 3 : 1;  // spectra per lane
+        const size_t per_step = (size_t)2 * nsp * sh.Mh * sizeof(cx<T>);
+        const size_t budget = (size_t)1 << 30;
+        int nb = (int)std::max<size_t>(1, budget / per_step);
+        if (nb > N) nb = N;
+        adj_nb = nb;
+        adj_N = N;
+        const int nblk = xdot_blocks(sh);
+        PO_HIP(adjS.alloc((size_t)2 * nsp * nb * sh.Mh));
+        PO_HIP(adjPart.alloc((size_t)N * nblk * 3));
+        PO_HIP(adjSum.alloc(3));
+        std::vector<PlaneDesc<T>> P1(2 * N), P2(rqm4 ? 2 * N : 0);
+        std::vector<XDesc<T>> X(rqm4 ? 2 * N : 0);
+        std::vector<DotDesc<T>> D(N);
+        const long M = sh.M;
+        for (int s = 0; s < N; ++s) {
+            const bool inA = s < nA;
+            const int t = N - 1 - s;
+            T *q[2] = {inA ? qA.p + (size_t)s * M : qB.p + (size_t)(s - nA) * M,
+                       t < nB ? qBc.p + (size_t)t * M : qAc.p + (size_t)(t - nB) * M};
+            const T *e2 = inA ? eA2.p : eB2.p, *e4 = inA ? eA4.p : eB4.p;
+            const KTable<T> &kf = inA ? kA : kB, &kh = inA ? kAh : kBh;
+            const int j = s % nb;
+            cx<T> *buf[2][3] = {};
+            for (int l = 0; l < 2; ++l)
+                for (int k = 0; k < nsp; ++k) buf[l][k] = adjS.p + ((size_t)(2 * j + l) * nsp + k) * sh.Mh;
+            for (int l = 0; l < 2; ++l) {
+                PlaneDesc<T> d{};
+                d.qin = q[l];
+                d.out0 = buf[l][0];
+                d.pre0 = e2;
+                if (rqm4) {
+                    d.out1 = buf[l][1];
+                    d.pre1 = e4;
+                    X[2 * s + l] = kh.desc(buf[l][1], buf[l][2]);  // K_h F E4 q
+                    PlaneDesc<T> m{};                               // F E4 S_h q, in place
+                    m.in0 = buf[l][2];
+                    m.post0 = e4;
+                    m.out0 = buf[l][2];
+                    m.pre0 = e4;
+                    P2[2 * s + l] = m;
+                }
+                P1[2 * s + l] = d;
+            }
+            DotDesc<T> dd{};
+            auto pair = [&](int p, const cx<T> *a, const cx<T> *b, const KTable<T> &kt, double c) {
+                dd.a[p] = a;
+                dd.b[p] = b;
+                dd.fx[p] = kt.fx.p;
+                dd.fy[p] = kt.fy.p;
+                dd.fz[p] = kt.fz.p;
+                dd.gx[p] = kt.gx.p;
+                dd.gy[p] = kt.gy.p;
+                dd.gz[p] = kt.gz.p;
+                dd.coef[p] = c;
+            };
+            if (rqm4) {
+                pair(0, buf[1][1], buf[0][2], kh, 4.0 / 3.0);
+                pair(1, buf[1][2], buf[0][1], kh, 4.0 / 3.0);
+                pair(2, buf[1][0], buf[0][0], kf, -1.0 / 3.0);
+                dd.npair = 3;
+            } else {
+                pair(0, buf[1][0], buf[0][0], kf, 1.0);
+                dd.npair = 1;
+            }
+            dd.partials = adjPart.p + (size_t)s * nblk * 3;
+            D[s] = dd;
+        }
+        PO_HIP(adjP1.alloc(P1.size()));
+        PO_HIP(hipMemcpyAsync(adjP1.p, P1.data(), sizeof(PlaneDesc<T>) * P1.size(), hipMemcpyHostToDevice, st));
+        if (rqm4) {
+            PO_HIP(adjP2.alloc(P2.size()));
+            PO_HIP(adjX.alloc(X.size()));
+            PO_HIP(hipMemcpyAsync(adjP2.p, P2.data(), sizeof(PlaneDesc<T>) * P2.size(), hipMemcpyHostToDevice, st));
+            PO_HIP(hipMemcpyAsync(adjX.p, X.data(), sizeof(XDesc<T>) * X.size(), hipMemcpyHostToDevice, st));
+        }
+        PO_HIP(adjD.alloc(D.size()));
+        PO_HIP(hipMemcpyAsync(adjD.p, D.data(), sizeof(DotDesc<T>) * D.size(), hipMemcpyHostToDevice, st));
+        PO_HIP(hipStreamSynchronize(st));
+        adj_ready = true;
+        return PO_OK;
+    }
+
+    // Enqueues the adjoint pass over the slices of the last propagator run;
+    // leaves sum_s sum_k (...) for d = 0..2 in adjSum (dQ/dl_d * M^2).
+    int adj_run() {
+        hipStream_t st = ctx->stream;
+        const bool rqm4 = desc.algo == PO_RQM4;
+        const int pv = prog.plane_variant;
+        for (int s0 = 0; s0 < adj_N; s0 += adj_nb) {
+            const int n = std::min(adj_nb, adj_N - s0);
+            hipError_t e = launch_plane<T>(sh, rt, adjP1.p + 2 * s0, 2 * n, st, pv);
+            if (e == hipSuccess && rqm4) e = launch_xpass<T>(sh, rt, adjX.p + 2 * s0, 2 * n, st);
+            if (e == hipSuccess && rqm4) e = launch_plane<T>(sh, rt, adjP2.p + 2 * s0, 2 * n, st, pv);
+            if (e == hipSuccess) e = launch_xdot<T>(sh, rt, adjD.p + s0, n, st);
+            if (e != hipSuccess) return fail(PO_ERR_DEVICE, std::string("adjoint launch: ") + hipGetErrorString(e));
+        }
+        PO_HIP(launch_sum_rows(adjPart.p, (long)adj_N * xdot_blocks(sh), 3, adjSum.p, st));
+        return PO_OK;
+    }
+
+    int stress(double *dHdl, double *Qout) override {
+        int rc;
+        if ((rc = adj_build())) return rc;
+        hipStream_t st = ctx->stream;
+        const long M = sh.M;
+        if ((rc = prog.run(sh, rt, st))) return rc;
+        PO_HIP(launch_mean_partials<T>(qB.p + (size_t)(sB - 1) * M, M, part.p, st));
+        PO_HIP(launch_finish_partials(part.p, kRedBlocks, 1, (double)M, adjSum.p, st));
+        double Q = 0.0;
+        PO_HIP(hipMemcpyAsync(&Q, adjSum.p, sizeof(double), hipMemcpyDeviceToHost, st));
+        if ((rc = adj_run())) return rc;
+        double sum[3];
+        PO_HIP(hipMemcpyAsync(sum, adjSum.p, sizeof(sum), hipMemcpyDeviceToHost, st));
+        PO_HIP(hipStreamSynchronize(st));
+        // H depends on the cell only through -ln Q at fixed fields
+        const double md = (double)M;
+        for (int d = 0; d < 3; ++d) dHdl[d] = -sum[d] / (md * md * Q);
+        if (Qout) *Qout = Q;
+        return PO_OK;
+    }
+
+    // New cell lengths / angles (same crystal system and grid): refills the
+    // K tables in place, so the chain programs keep their descriptors, and
+    // restarts the Anderson history from the current fields, as the
+    // reference's re-initialised model does (Model_AB::reset,
+    // src/Model_AB.cc:20-28; FieldAX ring w(n) = w0, src/FieldAX.cc:34-39).
+    int set_cell(const double *la) override {
+        po_grid_desc g = desc.grid;
+        g.a = la[0];
+        g.b = la[1];
+        g.c = la[2];
+        g.alpha = la[3];
+        g.beta = la[4];
+        g.gamma = la[5];
+        Cell nc{};
+        int rc;
+        if ((rc = make_cell(g, nc))) return rc;
+        if (nc.ortho != cell.ortho && desc.algo == PO_OS2)
+            return fail(PO_ERR_INVALID, "set_cell cannot switch an OS2 model between orthogonal and general cells");
+        hipStream_t st = ctx->stream;
+        PO_HIP(hipStreamSynchronize(st));
+        cell = nc;
+        desc.grid = g;
+        const bool general = (desc.algo == PO_OS2) && !cell.ortho;
+        if ((rc = build_ktable<T>(kA, sh, cell, general, dsA, st))) return rc;
+        if ((rc = build_ktable<T>(kB, sh, cell, general, dsB, st))) return rc;
+        if (desc.algo == PO_RQM4) {
+            if ((rc = build_ktable<T>(kAh, sh, cell, false, 0.5 * dsA, st))) return rc;
+            if ((rc = build_ktable<T>(kBh, sh, cell, false, 0.5 * dsB, st))) return rc;
+        }
+        cur = n;
+        k = 0;
+        PO_HIP(hipMemcpyAsync(wAh.p + (size_t)n * sh.M, wA.p, sizeof(double) * sh.M, hipMemcpyDeviceToDevice, st));
+        PO_HIP(hipMemcpyAsync(wBh.p + (size_t)n * sh.M, wB.p, sizeof(double) * sh.M, hipMemcpyDeviceToDevice, st));
+        PO_HIP(hipStreamSynchronize(st));
         return PO_OK;
     }
 
@@ -970,6 +1176,102 @@ int po_model_relax(po_model *m, int max_iter, int min_iter, int record_interval,
 }
 
 long po_model_contour_steps(const po_model *m) { return m ? m->steps : -1; }
+
+int po_model_stress(po_model *m, double *dHdl3, double *Q) {
+    if (!m || !dHdl3) return fail(PO_ERR_INVALID, "null argument");
+    PO_HIP(hipSetDevice(m->ctx->device));
+    return m->stress(dHdl3, Q);
+}
+
+// Free cell lengths of each crystal system and the length axes each one
+// scales (the angles stay fixed): mask[p] has bit d set when parameter p
+// (0 = a, 1 = b, 2 = c) is the length along axis d.
+static int free_lengths(int dim, int crystal, int *param, int *mask) {
+    if (dim == 1) { param[0] = 0; mask[0] = 1; return 1; }
+    if (dim == 2) {
+        if (crystal == 1 || crystal == 3) { param[0] = 0; mask[0] = 3; return 1; }  // SQUARE, HEXAGONAL
+        param[0] = 0; mask[0] = 1; param[1] = 1; mask[1] = 2;                          // RECTANGULAR, OBLIQUE
+        return 2;
+    }
+    switch (crystal) {
+    case 5: case 8: param[0] = 0; mask[0] = 7; return 1;                               // CUBIC, TRIGONAL
+    case 6: case 3: param[0] = 0; mask[0] = 3; param[1] = 2; mask[1] = 4; return 2;    // TETRAGONAL, HEXAGONAL
+    default: param[0] = 0; mask[0] = 1; param[1] = 1; mask[1] = 2; param[2] = 2; mask[2] = 4; return 3;
+    }
+}
+
+int po_model_relax_cell(po_model *m, int max_outer, double stress_tol, int max_iter, int min_iter,
+                        int record_interval, double thresh, double *cell6, double *H, double *stress3, int *outer) {
+    if (!m || max_outer < 1 || !(stress_tol > 0)) return fail(PO_ERR_INVALID, "bad argument");
+    PO_HIP(hipSetDevice(m->ctx->device));
+    const po_grid_desc g0 = m->desc.grid;
+    int param[3], mask[3];
+    const int np = free_lengths(g0.dim, g0.crystal, param, mask);
+    double len[3] = {g0.a, g0.b, g0.c};
+    double gp[3] = {0, 0, 0}, prev_len[3] = {0, 0, 0}, prev_g[3] = {0, 0, 0};
+    double dHdl[3] = {0, 0, 0}, h = 0.0, res = 0.0;
+    int k = 0, rc;
+    for (; k < max_outer; ++k) {
+        int it = 0;
+        if ((rc = po_model_relax(m, max_iter, min_iter, record_interval, thresh, &it, &h, &res))) return rc;
+        if (!std::isfinite(h)) return fail(PO_ERR_STATE, "SCFT diverged during cell relaxation");
+        if ((rc = m->stress(dHdl, nullptr))) return rc;
+        double gmax = 0.0;
+        for (int p = 0; p < np; ++p) {
+            gp[p] = 0.0;
+            for (int d = 0; d < 3; ++d)
+                if (mask[p] >> d & 1) gp[p] += dHdl[d];
+            gmax = std::max(gmax, std::fabs(gp[p]));
+        }
+        if (gmax < stress_tol) break;
+        if (k + 1 == max_outer) break;
+        // per-length secant on dH/dl (diagonal quasi-Newton); the first step
+        // and any non-convex secant fall back to a curvature guess of
+        // |H| / l^2, and no step exceeds 5% of the length
+        double nl[3];
+        for (int p = 0; p < np; ++p) {
+            const double l = len[param[p]];
+            double curv = std::fabs(h) / (l * l);
+            if (k > 0 && prev_len[p] != l) {
+                const double c = (gp[p] - prev_g[p]) / (l - prev_len[p]);
+                if (c > 0) curv = c;
+            }
+            double step = -gp[p] / curv;
+            const double cap = 0.05 * l;
+            if (step > cap) step = cap;
+            if (step < -cap) step = -cap;
+            prev_len[p] = l;
+            prev_g[p] = gp[p];
+            nl[p] = l + step;
+        }
+        for (int p = 0; p < np; ++p) len[param[p]] = nl[p];
+        // tied lengths follow their free parameter (b = a, c = a)
+        double a = len[0], b = len[1], c = len[2];
+        if (g0.dim == 2 && (g0.crystal == 1 || g0.crystal == 3)) b = a;
+        if (g0.dim == 3 && (g0.crystal == 5 || g0.crystal == 8)) b = c = a;
+        if (g0.dim == 3 && (g0.crystal == 6 || g0.crystal == 3)) b = a;
+        if ((rc = m->set_cell(std::array<double, 6>{a, b, c, g0.alpha, g0.beta, g0.gamma}.data()))) return rc;
+        len[0] = a;
+        len[1] = b;
+        len[2] = c;
+    }
+    const po_grid_desc &g = m->desc.grid;
+    if (cell6) {
+        cell6[0] = g.a; cell6[1] = g.b; cell6[2] = g.c;
+        cell6[3] = g.alpha; cell6[4] = g.beta; cell6[5] = g.gamma;
+    }
+    if (H) *H = h;
+    if (stress3) for (int p = 0; p < 3; ++p) stress3[p] = p < np ? gp[p] : 0.0;
+    if (outer) *outer = std::min(k + 1, max_outer);
+    return PO_OK;
+}
+
+int po_model_set_cell(po_model *m, double a, double b, double c, double alpha, double beta, double gamma) {
+    if (!m) return fail(PO_ERR_INVALID, "null model");
+    PO_HIP(hipSetDevice(m->ctx->device));
+    const double la[6] = {a, b, c, alpha, beta, gamma};
+    return m->set_cell(la);
+}
 
 int po_model_set_profiling(po_model *m, int on) {
     if (!m) return fail(PO_ERR_INVALID, "null model");

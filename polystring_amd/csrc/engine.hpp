@@ -35,13 +35,27 @@ template <typename T> struct PlaneDesc {
     cx<T> *out0, *out1;
 };
 
-// Per-lane descriptor of the x-pencil kernel: S <- invX(K * fwdX(S)) with
+// Per-lane descriptor of the x-pencil kernel: Sout <- invX(K * fwdX(S)) with
 // K(kx,ky,kz) = fx[kx] * fy[ky] * fz[kz] (orthogonal cells) or tab[...] (full
 // table, general cells).
 template <typename T> struct XDesc {
     cx<T> *S;
     const T *fx, *fy, *fz;
     const T *tab;
+    cx<T> *Sout;  // output spectrum; null writes back into S
+};
+
+// Per-step descriptor of the stress dot kernel: for each pair p,
+//   acc_d += coef[p] * sum_k w(kz) Re(conj(A_p(k)) B_p(k)) K_p(k) g_p,d(k_d)
+// over the full 3-D spectra of the yz spectra A_p, B_p (the x transform is
+// done in the kernel), with K_p = fx fy fz and dK/dL_d = K g_d (separable).
+template <typename T> struct DotDesc {
+    const cx<T> *a[3], *b[3];
+    const T *fx[3], *fy[3], *fz[3];
+    const T *gx[3], *gy[3], *gz[3];
+    double coef[3];
+    int npair;
+    double *partials;  // [xdot_blocks][3]
 };
 
 struct Shape {
@@ -55,6 +69,10 @@ template <typename T> struct Roots {
 };
 
 bool plane_supported(int Ly, int Lz);
+template <typename T>
+hipError_t launch_xdot(const Shape &sh, const Roots<T> &rt, const DotDesc<T> *d_desc, int ndesc, hipStream_t st);
+int xdot_blocks(const Shape &sh);  // workgroups per descriptor (partials per descriptor = 3x this)
+hipError_t launch_sum_rows(const double *partials, long nrows, int nval, double *out, hipStream_t st);
 bool xpass_supported(int Lx);
 
 // variant 0: default workgroup size; 1: 1024 threads (planes of >= 8192 reals)

@@ -134,6 +134,23 @@ hipError_t launch_finish_partials(const double *partials, int nblk, int nval, do
     return hipGetLastError();
 }
 
+// out[v] = sum_r partials[r * nval + v]: one workgroup per value, each
+// thread a fixed strided subset, then a fixed-order tree (deterministic).
+__global__ void __launch_bounds__(kRedThreads) k_sum_rows(const double *__restrict__ partials, long nrows, int nval,
+                                                          double *out) {
+    __shared__ double sh[kRedThreads / 64];
+    const int v = blockIdx.x;
+    double s = 0.0;
+    for (long r = threadIdx.x; r < nrows; r += kRedThreads) s += partials[r * nval + v];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) out[v] = s;
+}
+
+hipError_t launch_sum_rows(const double *partials, long nrows, int nval, double *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_sum_rows, dim3(nval), dim3(kRedThreads), 0, st, partials, nrows, nval, out);
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------------- Anderson --
 // d = (chiN phi_other + yita) - w with yita = (wA + wB - chiN)/2
 // (src/Model_AB.cc:232-238, src/FieldAX.cc:311-313), stored into the ring

@@ -477,7 +477,57 @@ k_xpass(const XDesc<T> *__restrict__ descs, Roots<T> rt, int ncol, int hz) {
     fft_lds<T, LX, true, NT, C, true>(buf, 1, C, rt.x);
     for (int idx = tid; idx < LX * C; idx += NT) {
         const int x = idx / C, col = c0 + idx % C;
-        if (col < ncol) d.S[(long)x * ncol + col] = buf[idx];
+        if (col < ncol) (d.Sout ? d.Sout : d.S)[(long)x * ncol + col] = buf[idx];
+    }
+}
+
+// Stress dot kernel for grids whose column count is not a multiple of the
+// tile (1-D and small 2-D grids): both x transforms in LDS.
+template <typename T, int LX>
+__global__ void __launch_bounds__((XCfg<T, LX>::NT))
+k_xdot(const DotDesc<T> *__restrict__ desc, Roots<T> rt, int ncol, int hz, int nyq) {
+    constexpr int C = XCfg<T, LX>::C, NT = XCfg<T, LX>::NT;
+    __shared__ cx<T> ba[LX * C], bb[LX * C];
+    __shared__ double red[3][NT / 64];
+    const DotDesc<T> d = desc[blockIdx.y];
+    const int c0 = blockIdx.x * C;
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int p = 0; p < d.npair; ++p) {
+        __syncthreads();
+        for (int idx = tid; idx < LX * C; idx += NT) {
+            const int x = idx / C, col = c0 + idx % C;
+            ba[idx] = col < ncol ? d.a[p][(long)x * ncol + col] : mk<T>((T)0, (T)0);
+            bb[idx] = col < ncol ? d.b[p][(long)x * ncol + col] : mk<T>((T)0, (T)0);
+        }
+        __syncthreads();
+        fft_lds<T, LX, false, NT, C, true>(ba, 1, C, rt.x);
+        fft_lds<T, LX, false, NT, C, true>(bb, 1, C, rt.x);
+        __syncthreads();
+        for (int idx = tid; idx < LX * C; idx += NT) {
+            const int x = idx / C, col = c0 + idx % C;
+            if (col < ncol) {
+                const int ky = col / hz, kz = col % hz;
+                const double wz = (kz == 0 || kz == nyq) ? 1.0 : 2.0;
+                const double re = (double)ba[idx].x * (double)bb[idx].x + (double)ba[idx].y * (double)bb[idx].y;
+                const double wk = d.coef[p] * wz * re * (double)d.fx[p][x] * (double)d.fy[p][ky] * (double)d.fz[p][kz];
+                acc[0] += wk * (double)d.gx[p][x];
+                acc[1] += wk * (double)d.gy[p][ky];
+                acc[2] += wk * (double)d.gz[p][kz];
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        double v = acc[k];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+        if (lane == 0) red[k][w] = v;
+    }
+    __syncthreads();
+    if (tid < 3) {
+        double s = 0.0;
+        for (int i = 0; i < NT / 64; ++i) s += red[tid][i];
+        d.partials[blockIdx.x * 3 + tid] = s;
     }
 }
 
@@ -535,8 +585,71 @@ k_xpass3(const XDesc<T> *__restrict__ descs, Roots<T> rt, int ncol, int hz) {
     }
     {
         cx<T> b[I::ITL][I::RL];
-        WaveFFT<T, LX, true, CPW, true, IO_REG, IO_GLB>::run(lds, 1, RS, rt.x, lane, nullptr, 1, ncol, d.S + cw, v,
-                                                             b);
+        WaveFFT<T, LX, true, CPW, true, IO_REG, IO_GLB>::run(lds, 1, RS, rt.x, lane, nullptr, 1, ncol,
+                                                             (d.Sout ? d.Sout : d.S) + cw, v, b);
+    }
+}
+
+// Stress dot kernel (see DotDesc): x transforms of both spectra of a pair in
+// registers, then the k-space weighted products, reduced per workgroup.
+template <typename T, int LX>
+__global__ void __launch_bounds__((X3Cfg<T, LX>::NT))
+k_xdot3(const DotDesc<T> *__restrict__ desc, Roots<T> rt, int ncol, int hz, int nyq) {
+    using X = X3Cfg<T, LX>;
+    constexpr int CPW = X::CPW, RS = X::RS, NT = X::NT;
+    using F = WaveEnds<LX, false, CPW>;
+    constexpr int RX = F::RL, ITX = F::ITL, NBX = LX / RX, TOTX = CPW * NBX;
+    __shared__ cx<T> buf[LX * RS];
+    __shared__ double red[3][NT / 64];
+    const DotDesc<T> d = desc[blockIdx.y];
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int cw = blockIdx.x * X::C + w * CPW;
+    cx<T> *lds = buf + w * CPW;
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int p = 0; p < d.npair; ++p) {
+        cx<T> va[ITX][RX], vb[ITX][RX];
+        {
+            cx<T> t[F::ITF][F::RF];
+            WaveFFT<T, LX, false, CPW, true, IO_GLB, IO_REG>::run(lds, 1, RS, rt.x, lane, d.a[p] + cw, 1, ncol,
+                                                                  nullptr, t, va);
+        }
+        {
+            cx<T> t[F::ITF][F::RF];
+            WaveFFT<T, LX, false, CPW, true, IO_GLB, IO_REG>::run(lds, 1, RS, rt.x, lane, d.b[p] + cw, 1, ncol,
+                                                                  nullptr, t, vb);
+        }
+#pragma unroll
+        for (int it = 0; it < ITX; ++it) {
+            const int b = lane + it * 64;
+            if ((TOTX % 64 == 0) || b < TOTX) {
+                const int c = cw + b % CPW, j = b / CPW;
+                const int ky = c / hz, kz = c % hz;
+                const double wz = (kz == 0 || kz == nyq) ? 1.0 : 2.0;
+                const double fyz = (double)gload(d.fy[p] + ky) * (double)gload(d.fz[p] + kz);
+                const double gy = gload(d.gy[p] + ky), gz = gload(d.gz[p] + kz);
+#pragma unroll
+                for (int r = 0; r < RX; ++r) {
+                    const int x = j + r * NBX;
+                    const double re = (double)va[it][r].x * (double)vb[it][r].x + (double)va[it][r].y * (double)vb[it][r].y;
+                    const double wk = d.coef[p] * wz * re * (double)gload(d.fx[p] + x) * fyz;
+                    acc[0] += wk * (double)gload(d.gx[p] + x);
+                    acc[1] += wk * gy;
+                    acc[2] += wk * gz;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        double v = acc[k];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+        if (lane == 0) red[k][w] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        double s = 0.0;
+        for (int i = 0; i < NT / 64; ++i) s += red[threadIdx.x][i];
+        d.partials[blockIdx.x * 3 + threadIdx.x] = s;
     }
 }
 
@@ -607,6 +720,36 @@ hipError_t launch_xpass(const Shape &sh, const Roots<T> &rt, const XDesc<T> *d_d
 #undef PO_CASE
     return hipErrorInvalidValue;
 }
+
+int xdot_blocks(const Shape &sh) {
+    const int ncol = sh.Ly * sh.Hz;
+    return ncol % 16 == 0 ? ncol / 16 : (ncol + 15) / 16;
+}
+
+template <typename T>
+hipError_t launch_xdot(const Shape &sh, const Roots<T> &rt, const DotDesc<T> *d_desc, int ndesc, hipStream_t st) {
+    const int ncol = sh.Ly * sh.Hz;
+    const int nyq = (sh.Lz > 1 && sh.Lz % 2 == 0) ? sh.Lz / 2 : -1;
+    static_assert(X3Cfg<T, 8>::C == 16 && XCfg<T, 8>::C == 16, "xdot_blocks assumes 16-column tiles");
+#define PO_CASE(a)                                                                                          \
+    if (sh.Lx == a) {                                                                                       \
+        if (ncol % X3Cfg<T, a>::C == 0)                                                                     \
+            hipLaunchKernelGGL((k_xdot3<T, a>), dim3(ncol / X3Cfg<T, a>::C, ndesc), dim3(X3Cfg<T, a>::NT), 0, st, \
+                               d_desc, rt, ncol, sh.Hz, nyq);                                               \
+        else                                                                                                \
+            hipLaunchKernelGGL((k_xdot<T, a>), dim3((ncol + XCfg<T, a>::C - 1) / XCfg<T, a>::C, ndesc),        \
+                               dim3(XCfg<T, a>::NT), 0, st, d_desc, rt, ncol, sh.Hz, nyq);                 \
+        return hipGetLastError();                                                                           \
+    }
+    PO_X_SIZES(PO_CASE)
+#undef PO_CASE
+    return hipErrorInvalidValue;
+}
+
+template hipError_t launch_xdot<double>(const Shape &, const Roots<double> &, const DotDesc<double> *, int,
+                                        hipStream_t);
+template hipError_t launch_xdot<float>(const Shape &, const Roots<float> &, const DotDesc<float> *, int,
+                                       hipStream_t);
 
 template hipError_t launch_plane<double>(const Shape &, const Roots<double> &, const PlaneDesc<double> *, int,
                                          hipStream_t, int);

@@ -1,0 +1,146 @@
+"""GPU adjoint: cell stress dH/dl through the C ABI against central finite
+differences of the oracle's -ln Q, and stress-driven cell relaxation against
+the reference's recorded Brent optimum (polyorder-master
+example/AB/LAM/1D/cell_opt_rqm4_anderson.mat).
+
+Bars: FP64 stress to 1e-7 relative to the finite difference (dl = 1e-4, whose
+own truncation error is ~1e-10 here); FP32-bulk stress to 1e-4.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from polystring_amd import capi
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def smooth_field(shape, seed, amp=3.0, offset=1.0):
+    rng = np.random.default_rng(seed)
+    w = rng.standard_normal(shape)
+    W = np.fft.fftn(w)
+    k2 = sum(np.meshgrid(*[np.minimum(np.arange(n), n - np.arange(n)) ** 2 for n in shape], indexing="ij"))
+    W *= np.exp(-0.05 * k2)
+    w = np.real(np.fft.ifftn(W))
+    return amp * w / (np.abs(w).max() + 1e-300) + offset
+
+
+def oracle_lnQ(wA, wB, dim, crystal, lengths, Ms, algo):
+    """ln Q of the chain qA -> qB on the oracle (src/Model_AB.cc:139-145,195)."""
+    shape = wA.shape
+    N = Ms[0] - 1 + Ms[1] - 1
+    ds = 1.0 / N
+    l, g = O.cell_shape(dim, crystal, *lengths)
+    if algo == "RQM4":
+        k2 = O.k2_orthogonal(dim, l, shape)
+        qA = O.rqm4_solve(k2, ds, wA, np.ones(shape), Ms[0])
+        qB = O.rqm4_solve(k2, ds, wB, qA[-1], Ms[1])
+    else:
+        lap = O.klaplacian(g, shape, ds)
+        qA = O.os2_solve(lap, ds, wA, np.ones(shape), Ms[0])
+        qB = O.os2_solve(lap, ds, wB, qA[-1], Ms[1])
+    return float(np.log(qB[-1].mean()))
+
+
+def fd_stress(wA, wB, dim, crystal, lengths, Ms, algo, axes, h=1e-4):
+    """-(d ln Q / dl) by central differences; `axes` lists which lengths move together."""
+    lp = list(lengths)
+    lm = list(lengths)
+    for a in axes:
+        lp[a] += h
+        lm[a] -= h
+    return -(oracle_lnQ(wA, wB, dim, crystal, lp, Ms, algo) - oracle_lnQ(wA, wB, dim, crystal, lm, Ms, algo)) / (2 * h)
+
+
+# (shape, crystal, lengths, dim); lengths are a, b, c
+CASES = [
+    ((32, 32, 32), "ORTHORHOMBIC", (3.8, 4.0, 4.3), 3),
+    ((16, 16, 16), "ORTHORHOMBIC", (3.1, 3.5, 4.2), 3),
+    ((64, 1, 1), "LAMELLAR", (4.2, 0.0, 0.0), 1),
+    ((32, 32, 1), "RECTANGULAR", (3.8, 4.3, 0.0), 2),
+    ((16, 8, 1), "RECTANGULAR", (3.3, 2.9, 0.0), 2),
+]
+
+
+@pytest.mark.parametrize("shape,crystal,lengths,dim", CASES)
+@pytest.mark.parametrize("algo", ["RQM4", "OS2"])
+def test_stress_matches_finite_differences(ctx64, shape, crystal, lengths, dim, algo):
+    print(f"\n[stress] {algo} {shape} {crystal} {lengths}", flush=True)
+    wA = smooth_field(shape, 21, offset=2.0)
+    wB = smooth_field(shape, 22, offset=-1.0)
+    Ms = (13, 19)
+    m = capi.ModelAB(ctx64, wA, wB, cell=(crystal,) + tuple(lengths), Ms=Ms, chiN=15.0, algo=algo, dim=dim,
+                     n_anderson=3)
+    g, Q = m.stress()
+    lnQ = oracle_lnQ(wA, wB, dim, crystal, lengths, Ms, algo)
+    assert abs(np.log(Q) - lnQ) <= 1e-12 * max(1.0, abs(lnQ))
+    for d in range(3):
+        if d >= dim:
+            assert g[d] == 0.0
+            continue
+        fd = fd_stress(wA, wB, dim, crystal, lengths, Ms, algo, [d])
+        print(f"  d={d}: adjoint {g[d]!r} fd {fd!r}", flush=True)
+        assert abs(g[d] - fd) <= 1e-7 * abs(fd) + 1e-12
+
+
+def test_cubic_stress_is_axis_sum(ctx64):
+    """A cubic cell has one length: dH/da = sum of the three axis stresses."""
+    shape, Ms = (32, 32, 32), (21, 41)
+    wA = smooth_field(shape, 31, offset=2.0)
+    wB = smooth_field(shape, 32, offset=-1.0)
+    m = capi.ModelAB(ctx64, wA, wB, cell=("CUBIC", 4.1), Ms=Ms, chiN=15.0, n_anderson=3)
+    g, _ = m.stress()
+    fd = fd_stress(wA, wB, 3, "CUBIC", (4.1, 0.0, 0.0), Ms, "RQM4", [0])
+    assert abs(g.sum() - fd) <= 1e-7 * abs(fd)
+
+
+def test_stress_after_set_cell(ctx64):
+    """set_cell refills the k tables in place: the model's Q and stress follow the new cell."""
+    shape, Ms = (16, 16, 16), (13, 19)
+    wA = smooth_field(shape, 41, offset=2.0)
+    wB = smooth_field(shape, 42, offset=-1.0)
+    m = capi.ModelAB(ctx64, wA, wB, cell=("ORTHORHOMBIC", 3.8, 4.0, 4.3), Ms=Ms, chiN=15.0, n_anderson=3)
+    m.stress()
+    m.set_cell(3.6, 4.4, 4.1)
+    g, Q = m.stress()
+    lengths = (3.6, 4.4, 4.1)
+    assert abs(np.log(Q) - oracle_lnQ(wA, wB, 3, "ORTHORHOMBIC", lengths, Ms, "RQM4")) <= 1e-12
+    for d in range(3):
+        fd = fd_stress(wA, wB, 3, "ORTHORHOMBIC", lengths, Ms, "RQM4", [d])
+        assert abs(g[d] - fd) <= 1e-7 * abs(fd)
+
+
+def test_stress_fp32_bulk(ctx32):
+    shape, Ms = (32, 32, 32), (21, 41)
+    lengths = (3.8, 4.0, 4.3)
+    wA = smooth_field(shape, 51, offset=2.0)
+    wB = smooth_field(shape, 52, offset=-1.0)
+    m = capi.ModelAB(ctx32, wA, wB, cell=("ORTHORHOMBIC",) + lengths, Ms=Ms, chiN=15.0, n_anderson=3)
+    g, _ = m.stress()
+    for d in range(3):
+        fd = fd_stress(wA, wB, 3, "ORTHORHOMBIC", lengths, Ms, "RQM4", [d])
+        print(f"\n  fp32 d={d}: adjoint {g[d]!r} fd {fd!r} rel {abs(g[d] - fd) / abs(fd):.2e}", flush=True)
+        assert abs(g[d] - fd) <= 1e-4 * abs(fd)
+
+
+def test_cell_relaxation_reaches_brent_optimum(ctx64):
+    """1-D LAM (chiN=30, f=0.5, Ms=[51,51], RQM4 + Anderson), started from the
+    converged a = 4.0 field: the stress-driven relaxation lands on the
+    reference's Brent optimum a* = 4.476742 (F = 4.745625804830262), within the
+    Brent bracket's resolution, at an F no higher than Brent's best."""
+    known = json.load(open(os.path.join(GOLD, "known_answers.json")))["1D_brent_rqm4_anderson"]
+    i = int(np.argmin(known["F"]))
+    a_star, F_star = known["a"][i], known["F"][i]
+    f = np.load(os.path.join(GOLD, "lam1d_field.npz"))
+    m = capi.ModelAB(ctx64, f["wA"], f["wB"], cell=("LAMELLAR", 4.0), Ms=(51, 51), chiN=30.0,
+                     lam=(0.05, 0.05), n_anderson=5, dim=1)
+    cell, H, st, outer = m.relax_cell(max_outer=30, stress_tol=1e-7, max_iter=3000, thresh=1e-10)
+    print(f"\n[cell] a {cell[0]!r} H {H!r} stress {st[0]:.2e} outer steps {outer}", flush=True)
+    assert abs(st[0]) < 1e-7
+    assert abs(cell[0] - a_star) < 3e-4
+    assert H <= F_star + 1e-9
+    assert H >= F_star - 1e-8

@@ -144,6 +144,15 @@ long po_model_contour_steps(const po_model *m);
  * separable k tables: RQM4 on any cell (it uses the orthogonal k^2,
  * src/RQM4.cc:9-16), OS2 on orthogonal cells. */
 int po_model_stress(po_model *m, double *dHdl3, double *Q);
+/* Fused adjoint: with on != 0 every later po_model_update also computes the
+ * cell stress of the fields it solved with, inside the propagator passes
+ * (spectra of the first half of the chains are kept and dotted with the
+ * second half's), at no extra transform; po_model_last_stress returns it.
+ * No reference counterpart (the reference has no adjoint).  Needs
+ * Ly*(Lz/2+1) divisible by 16 and an even number of contour steps;
+ * po_model_stress covers every grid. */
+int po_model_set_adjoint(po_model *m, int on);
+int po_model_last_stress(po_model *m, double *dHdl3);
 /* New cell lengths and angles (radians; same crystal system and grid), as a
  * reference cell-optimisation step re-initialises UnitCell and the updaters
  * (src/scft.cc:361-380).  The fields are kept and the Anderson history

@@ -32,7 +32,7 @@ EXPORTS = [
     "po_prop_set_head_from_tail", "po_prop_solve", "po_prop_get_slice", "po_prop_Q", "po_density",
     "po_model_create", "po_model_destroy", "po_model_update", "po_model_observe", "po_model_get",
     "po_model_relax", "po_model_contour_steps", "po_model_stress", "po_model_set_cell",
-    "po_model_relax_cell",
+    "po_model_relax_cell", "po_model_set_adjoint", "po_model_last_stress",
     "po_model_set_profiling", "po_model_profile",
 ]
 
@@ -86,6 +86,7 @@ def load():
         "po_model_contour_steps": ([_vp], ctypes.c_long),
         "po_model_stress": ([_vp, _dp, _dp], i), "po_model_set_cell": ([_vp] + [d] * 6, i),
         "po_model_relax_cell": ([_vp, i, d, i, i, i, d, _dp, _dp, _dp, ctypes.POINTER(i)], i),
+        "po_model_set_adjoint": ([_vp, i], i), "po_model_last_stress": ([_vp, _dp], i),
         "po_model_set_profiling": ([_vp, i], i), "po_model_profile": ([_vp, _dp], i),
     }
     for name, (args, res) in sig.items():
@@ -259,6 +260,16 @@ class ModelAB:
                                           int(record_interval), float(thresh), _p(cell), _p(H), _p(st),
                                           ctypes.byref(k)))
         return cell, float(H[0]), st, k.value
+
+    def set_adjoint(self, on=True):
+        """Fused per-update stress (see last_stress)."""
+        _check(load().po_model_set_adjoint(self.h, 1 if on else 0))
+
+    def last_stress(self):
+        """dH/dl of the fields the last update solved with (fused adjoint on)."""
+        g = np.zeros(3)
+        _check(load().po_model_last_stress(self.h, _p(g)))
+        return g
 
     def set_cell(self, a, b=0.0, c=0.0, alpha=0.0, beta=0.0, gamma=0.0):
         _check(load().po_model_set_cell(self.h, float(a), float(b), float(c), float(alpha), float(beta),

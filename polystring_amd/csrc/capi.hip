@@ -293,6 +293,20 @@ template <typename T> struct Lane {
     cx<T> *Sf, *Sh;
 };
 
+// Fused adjoint stress of a two-lane program (lane 0 the forward chain,
+// lane 1 the complementary chain, both N steps with N even).  Step s of the
+// forward chain pairs with step N-1-s of the complementary chain.  In the
+// first half the x-pencil passes store each lane's three full spectra per step
+// (F E2 q, F E4 q, F E4 S_h q); in the second half each pass dots its live
+// spectrum with the partner's stored one (kinds 0-0, 1-2, 2-1), so every
+// step pair is visited exactly once and no spectrum is recomputed.
+template <typename T> struct AdjointArena {
+    cx<T> *spec = nullptr;        // (N/2) x 2 lanes x kinds (3 RQM4, 1 OS2) x elems
+    long elems = 0;               // complex per stored spectrum
+    double *partials = nullptr;   // per dot descriptor: nblk x 3
+    int nblk = 0;
+};
+
 template <typename T> struct Program {
     struct Launch {
         bool plane;
@@ -301,8 +315,9 @@ template <typename T> struct Program {
     DBuf<PlaneDesc<T>> pd;
     DBuf<XDesc<T>> xd;
     std::vector<Launch> seq;
+    int ndot = 0;  // dot descriptors of a fused adjoint program
 
-    int build(int algo, const std::vector<Lane<T>> &lanes, hipStream_t st) {
+    int build(int algo, const std::vector<Lane<T>> &lanes, hipStream_t st, const AdjointArena<T> *adj = nullptr) {
         const int L = (int)lanes.size();
         int nsteps = -1;
         for (const auto &ln : lanes) {
@@ -325,6 +340,28 @@ template <typename T> struct Program {
         auto push_p = [&](std::vector<PlaneDesc<T>> &v) { seq.push_back({true, (int)P.size(), (int)v.size()}); P.insert(P.end(), v.begin(), v.end()); };
         auto push_x = [&](std::vector<XDesc<T>> &v) { seq.push_back({false, (int)X.size(), (int)v.size()}); X.insert(X.end(), v.begin(), v.end()); };
         const bool rqm4 = algo == PO_RQM4;
+        if (adj && (L != 2 || nsteps % 2 != 0))
+            return fail(PO_ERR_INVALID, "the fused adjoint needs two lanes with an even step count");
+        ndot = 0;
+        // attach the adjoint store / dot of (step t, lane l, kind k) to an x-pass descriptor
+        auto adjoin = [&](XDesc<T> x, const KTable<T> &kt, int t, int l, int k) {
+            if (!adj) return x;
+            const int half = nsteps / 2, nk = rqm4 ? 3 : 1;
+            x.gx = kt.gx.p;
+            x.gy = kt.gy.p;
+            x.gz = kt.gz.p;
+            if (t < half) {
+                x.Sstore = adj->spec + ((size_t)(t * 2 + l) * nk + k) * adj->elems;
+            } else {
+                static const int partner[3] = {0, 2, 1};
+                const int tp = nsteps - 1 - t;
+                x.Sdot = adj->spec + ((size_t)(tp * 2 + (1 - l)) * nk + partner[k]) * adj->elems;
+                x.coef = rqm4 ? (k == 0 ? -1.0 / 3.0 : 4.0 / 3.0) : 1.0;
+                x.partials = adj->partials + (size_t)ndot * adj->nblk * 3;
+                ++ndot;
+            }
+            return x;
+        };
         {
             std::vector<PlaneDesc<T>> v;
             for (int l = 0; l < L; ++l) {
@@ -349,19 +386,19 @@ template <typename T> struct Program {
                 f.in0 = lanes[l].Sf;
                 f.post0 = sg.e2;
                 if (rqm4) {
-                    x2.push_back(sg.kfull->desc(lanes[l].Sf));
-                    x2.push_back(sg.khalf->desc(lanes[l].Sh));
+                    x2.push_back(adjoin(sg.kfull->desc(lanes[l].Sf), *sg.kfull, t, l, 0));
+                    x2.push_back(adjoin(sg.khalf->desc(lanes[l].Sh), *sg.khalf, t, l, 1));
                     PlaneDesc<T> m{};
                     m.in0 = lanes[l].Sh;
                     m.post0 = sg.e4;
                     m.out0 = lanes[l].Sh;
                     m.pre0 = sg.e4;
                     mid.push_back(m);
-                    x1.push_back(sg.khalf->desc(lanes[l].Sh));
+                    x1.push_back(adjoin(sg.khalf->desc(lanes[l].Sh), *sg.khalf, t, l, 2));
                     f.in1 = lanes[l].Sh;
                     f.post1 = sg.e4;
                 } else {
-                    x2.push_back(sg.kfull->desc(lanes[l].Sf));
+                    x2.push_back(adjoin(sg.kfull->desc(lanes[l].Sf), *sg.kfull, t, l, 0));
                 }
                 f.qout = sg.q + (size_t)(s + 1) * slice_elems;
                 if (s + 1 == sg.len - 1 && si + 1 < (int)lanes[l].segs.size()) f.qout2 = lanes[l].segs[si + 1].q;
@@ -597,6 +634,8 @@ struct po_model {
     virtual int observe(double *out) = 0;
     virtual int get(int which, double *out) = 0;
     virtual int stress(double *dHdl, double *Q) = 0;
+    virtual int set_adjoint(bool on) = 0;
+    virtual int last_stress(double *dHdl) = 0;
     virtual int set_cell(const double *lengths_angles) = 0;
 };
 
@@ -615,6 +654,7 @@ template <typename T> struct Model : po_model {
     DBuf<cx<T>> S[4];
     KTable<T> kA, kAh, kB, kBh;
     Program<T> prog;
+    std::vector<Lane<T>> lanes_;
 
     int init(po_ctx *c, const po_model_desc &d, const double *wA0, const double *wB0) {
         ctx = c;
@@ -707,6 +747,7 @@ template <typename T> struct Model : po_model {
         lanes[1].Sf = S[2].p;
         lanes[1].Sh = S[3].p;
         prog.slice_elems = M;
+        lanes_ = lanes;
         if ((rc = prog.build(d.algo, lanes, st))) return rc;
         return prog.tune(sh, rt, st);
     }
@@ -722,8 +763,14 @@ template <typename T> struct Model : po_model {
         const long M = sh.M;
         for (int it = 0; it < iters; ++it) {
             PO_HIP(mark());
-            int rc = prog.run_graph(sh, rt, st);
+            int rc = (fused_on ? prog_adj : prog).run_graph(sh, rt, st);
             if (rc) return rc;
+            if (fused_on) {
+                PO_HIP(launch_sum_rows(fusedPart.p, (long)prog_adj.ndot * xdot_blocks(sh), 3, fusedSum.p, st));
+                fused_valid = true;
+            } else {
+                fused_valid = false;
+            }
             PO_HIP(mark());
             // Q = qB->Qt() (src/Model_AB.cc:195)
             PO_HIP(launch_mean_partials<T>(qB.p + (size_t)(sB - 1) * M, M, part.p, st));
@@ -781,6 +828,59 @@ template <typename T> struct Model : po_model {
     //   (F E2 v, F E2 u) with dK_f, weight -1/3;  OS2 (A = S_f) the last with weight 1.
     // Steps are independent given the stored slices, so they run in batches
     // of adj_nb steps, two lanes (u, v) per step, per launch.
+    // fused per-update stress (see AdjointArena)
+    Program<T> prog_adj;
+    DBuf<cx<T>> fusedSpec;
+    DBuf<double> fusedPart, fusedSum;
+    bool fused_on = false, fused_valid = false;
+
+    int set_adjoint(bool on) override {
+        if (!on) {
+            fused_on = false;
+            return PO_OK;
+        }
+        if (!prog_adj.seq.empty()) {
+            fused_on = true;
+            return PO_OK;
+        }
+        const bool rqm4 = desc.algo == PO_RQM4;
+        const long elems = xstore_elems<T>(sh);
+        const int N = (sA - 1) + (sB - 1);
+        if (elems < 0 || N % 2 != 0 || (!rqm4 && !cell.ortho))
+            return fail(PO_ERR_INVALID, "the fused adjoint needs Ly*(Lz/2+1) divisible by 16, an even number of "
+                                        "contour steps and separable k tables; po_model_stress works for every grid");
+        hipStream_t st = ctx->stream;
+        const int nk = rqm4 ? 3 : 1;
+        AdjointArena<T> a;
+        a.elems = elems;
+        a.nblk = xdot_blocks(sh);
+        PO_HIP(fusedSpec.alloc((size_t)(N / 2) * 2 * nk * elems));
+        PO_HIP(fusedPart.alloc((size_t)(N / 2) * 2 * nk * a.nblk * 3));
+        PO_HIP(fusedSum.alloc(3));
+        a.spec = fusedSpec.p;
+        a.partials = fusedPart.p;
+        prog_adj.slice_elems = sh.M;
+        int rc = prog_adj.build(desc.algo, lanes_, st, &a);
+        if (rc) {
+            prog_adj.seq.clear();
+            return rc;
+        }
+        prog_adj.plane_variant = prog.plane_variant;
+        fused_on = true;
+        return PO_OK;
+    }
+
+    int last_stress(double *dHdl) override {
+        if (!fused_valid) return fail(PO_ERR_STATE, "no update has run with the fused adjoint on");
+        double o[4];
+        PO_HIP(hipMemcpyAsync(o, fusedSum.p, sizeof(double) * 3, hipMemcpyDeviceToHost, ctx->stream));
+        PO_HIP(hipMemcpyAsync(o + 3, obs.p + OBS_Q, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        PO_HIP(hipStreamSynchronize(ctx->stream));
+        const double md = (double)sh.M;
+        for (int d = 0; d < 3; ++d) dHdl[d] = -o[d] / (md * md * o[3]);
+        return PO_OK;
+    }
+
     DBuf<cx<T>> adjS;
     DBuf<PlaneDesc<T>> adjP1, adjP2;
     DBuf<XDesc<T>> adjX;
@@ -1264,6 +1364,17 @@ int po_model_relax_cell(po_model *m, int max_outer, double stress_tol, int max_i
     if (stress3) for (int p = 0; p < 3; ++p) stress3[p] = p < np ? gp[p] : 0.0;
     if (outer) *outer = std::min(k + 1, max_outer);
     return PO_OK;
+}
+
+int po_model_set_adjoint(po_model *m, int on) {
+    if (!m) return fail(PO_ERR_INVALID, "null model");
+    PO_HIP(hipSetDevice(m->ctx->device));
+    return m->set_adjoint(on != 0);
+}
+
+int po_model_last_stress(po_model *m, double *dHdl3) {
+    if (!m || !dHdl3) return fail(PO_ERR_INVALID, "null argument");
+    return m->last_stress(dHdl3);
 }
 
 int po_model_set_cell(po_model *m, double a, double b, double c, double alpha, double beta, double gamma) {

@@ -43,6 +43,15 @@ template <typename T> struct XDesc {
     const T *fx, *fy, *fz;
     const T *tab;
     cx<T> *Sout;  // output spectrum; null writes back into S
+    // Fused adjoint stress (x-pencil kernel with full tiles only, separable K):
+    //   Sstore: store fwdX(S) (register order, xstore_elems complex);
+    //   Sdot:   a partner's stored spectrum Y; partials[blk*3 + d] =
+    //           coef * sum_k w(kz) Re(conj(Y) fwdX(S)) K g_d(k_d).
+    cx<T> *Sstore;
+    const cx<T> *Sdot;
+    const T *gx, *gy, *gz;
+    double coef;
+    double *partials;
 };
 
 // Per-step descriptor of the stress dot kernel: for each pair p,
@@ -71,7 +80,10 @@ template <typename T> struct Roots {
 bool plane_supported(int Ly, int Lz);
 template <typename T>
 hipError_t launch_xdot(const Shape &sh, const Roots<T> &rt, const DotDesc<T> *d_desc, int ndesc, hipStream_t st);
-int xdot_blocks(const Shape &sh);  // workgroups per descriptor (partials per descriptor = 3x this)
+int xdot_blocks(const Shape &sh);
+// complex elements of one stored spectrum of the fused adjoint, -1 when the
+// grid has no full-tile x-pencil kernel
+template <typename T> long xstore_elems(const Shape &sh);  // workgroups per descriptor (partials per descriptor = 3x this)
 hipError_t launch_sum_rows(const double *partials, long nrows, int nval, double *out, hipStream_t st);
 bool xpass_supported(int Lx);
 

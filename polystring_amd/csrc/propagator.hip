@@ -569,6 +569,17 @@ k_xpass3(const XDesc<T> *__restrict__ descs, Roots<T> rt, int ncol, int hz) {
         WaveFFT<T, LX, false, CPW, true, IO_GLB, IO_REG>::run(lds, 1, RS, rt.x, lane, d.S + cw, 1, ncol, nullptr, a,
                                                               v);
     }
+    // full 3-D spectrum of this wave's columns, in register order (stored
+    // for, or read from, the partner chain of the adjoint stress)
+    const unsigned sbase = (blockIdx.x * X::NW + w) * (unsigned)(ITX * RX * 64);
+    if (d.Sstore) {
+#pragma unroll
+        for (int it = 0; it < ITX; ++it)
+            if ((TOTX % 64 == 0) || lane + it * 64 < TOTX)
+#pragma unroll
+                for (int r = 0; r < RX; ++r) gstore(d.Sstore + sbase + (unsigned)((it * RX + r) * 64 + lane), v[it][r]);
+    }
+    double acc[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int it = 0; it < ITX; ++it) {
         const int b = lane + it * 64;
@@ -579,8 +590,33 @@ k_xpass3(const XDesc<T> *__restrict__ descs, Roots<T> rt, int ncol, int hz) {
                 const int x = j + r * NBX;
                 const T f = d.tab ? gload(d.tab + (unsigned)(x * ncol + c))
                                   : gload(d.fx + x) * gload(d.fy + c / hz) * gload(d.fz + c % hz);
+                if (d.Sdot) {
+                    // sum_k w(kz) Re(conj(Y) X) K dK/dl_d / K, X = this chain, Y = partner
+                    const cx<T> y = gload(d.Sdot + sbase + (unsigned)((it * RX + r) * 64 + lane));
+                    const int ky = c / hz, kz = c % hz;
+                    const double wz = (kz == 0 || kz == hz - 1) ? 1.0 : 2.0;
+                    const double wk = wz * (double)f * ((double)y.x * (double)v[it][r].x + (double)y.y * (double)v[it][r].y);
+                    acc[0] += wk * (double)gload(d.gx + x);
+                    acc[1] += wk * (double)gload(d.gy + ky);
+                    acc[2] += wk * (double)gload(d.gz + kz);
+                }
                 v[it][r] = scl<T>(v[it][r], f);
             }
+        }
+    }
+    if (d.Sdot) {
+        __shared__ double red[3][X::NW];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            double a = acc[k];
+            for (int off = 32; off > 0; off >>= 1) a += __shfl_down(a, off, 64);
+            if (lane == 0) red[k][w] = a;
+        }
+        __syncthreads();
+        if (threadIdx.x < 3) {
+            double sum = 0.0;
+            for (int i = 0; i < X::NW; ++i) sum += red[threadIdx.x][i];
+            d.partials[blockIdx.x * 3 + threadIdx.x] = d.coef * sum;
         }
     }
     {
@@ -720,6 +756,21 @@ hipError_t launch_xpass(const Shape &sh, const Roots<T> &rt, const XDesc<T> *d_d
 #undef PO_CASE
     return hipErrorInvalidValue;
 }
+
+template <typename T> long xstore_elems(const Shape &sh) {
+    const int ncol = sh.Ly * sh.Hz;
+#define PO_CASE(a)                                                                                  \
+    if (sh.Lx == a) {                                                                               \
+        using X = X3Cfg<T, a>;                                                                      \
+        using F = WaveEnds<a, false, X::CPW>;                                                       \
+        return ncol % X::C ? -1L : (long)(ncol / X::C) * X::NW * F::ITL * F::RL * 64;               \
+    }
+    PO_X_SIZES(PO_CASE)
+#undef PO_CASE
+    return -1;
+}
+template long xstore_elems<double>(const Shape &);
+template long xstore_elems<float>(const Shape &);
 
 int xdot_blocks(const Shape &sh) {
     const int ncol = sh.Ly * sh.Hz;

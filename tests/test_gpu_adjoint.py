@@ -144,3 +144,43 @@ def test_cell_relaxation_reaches_brent_optimum(ctx64):
     assert abs(cell[0] - a_star) < 3e-4
     assert H <= F_star + 1e-9
     assert H >= F_star - 1e-8
+
+
+@pytest.mark.parametrize("algo", ["RQM4", "OS2"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_fused_stress_matches_standalone(ctx64, ctx32, algo, precision):
+    """The stress fused into update() (spectra of the first half of the chains
+    kept and dotted with the second half's) equals the standalone adjoint pass
+    at the same fields, and leaves the forward results bit-identical."""
+    ctx = ctx64 if precision == "fp64" else ctx32
+    shape, Ms = (32, 32, 32), (13, 19)
+    lengths = (3.8, 4.0, 4.3)
+    wA = smooth_field(shape, 61, offset=2.0)
+    wB = smooth_field(shape, 62, offset=-1.0)
+    kw = dict(cell=("ORTHORHOMBIC",) + lengths, Ms=Ms, chiN=15.0, algo=algo, n_anderson=3)
+    ref = capi.ModelAB(ctx, wA, wB, **kw)
+    g_ref, _ = ref.stress()
+    fused = capi.ModelAB(ctx, wA, wB, **kw)
+    fused.set_adjoint(True)
+    plain = capi.ModelAB(ctx, wA, wB, **kw)
+    for it in range(3):
+        fused.update()
+        plain.update()
+        g = fused.last_stress()
+        if it == 0:
+            tol = 1e-12 if precision == "fp64" else 1e-5
+            print(f"\n  fused {g} standalone {g_ref}", flush=True)
+            assert np.abs(g - g_ref).max() <= tol * np.abs(g_ref).max()
+        a, b = fused.observe(), plain.observe()
+        assert a["H"] == b["H"] and a["Q"] == b["Q"] and a["residual"] == b["residual"]
+    assert np.array_equal(fused.get("wA"), plain.get("wA"))
+
+
+def test_fused_adjoint_rejects_partial_tiles(ctx64):
+    """Grids whose x-pencil columns do not fill 16-column tiles use the standalone pass."""
+    f = np.load(os.path.join(GOLD, "lam1d_field.npz"))
+    m = capi.ModelAB(ctx64, f["wA"], f["wB"], cell=("LAMELLAR", 4.0), Ms=(51, 51), chiN=30.0, dim=1)
+    with pytest.raises(capi.POError):
+        m.set_adjoint(True)
+    with pytest.raises(capi.POError):
+        m.last_stress()

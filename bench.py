@@ -6,12 +6,20 @@ f_A = 0.35, chiN = 18, cubic cell a = 4.2, 128^3 grid, Ns = 200
 (history 20, lam 0.1), seeded double-gyroid template + noise.  One "step" is
 one full SCFT iteration (Model_AB::update: four propagators, Q, densities,
 Anderson mixing) with every input resident in HBM; it advances 2*Ns = 400
-contour steps.
+contour steps (adjoint work is not counted as contour steps).
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64|fp32]
+Two forms are timed in the same run:
+  * mixed + adjoint (the headline `value`): FP32 propagators with FP64 Q,
+    H, Gram sums and Anderson solve, and the cell-stress adjoint fused into
+    every update;
+  * FP64 plain: the same iteration in FP64 without the adjoint.
+The precision error is measured on the same fields: H, phi_A, phi_B and the
+residual after the first update from the same initial fields, FP32 against
+FP64.
 
-Prints one JSON line.  value = whole-job contour-steps/s of the plain FP64
-path; the FP32-bulk form is timed in the same run and reported beside it.
+  python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Prints one JSON line.
 """
 import argparse
 import json
@@ -26,12 +34,22 @@ HBM_PEAK_GBS = 8000.0
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "traffic_rqm4_step_128cubed.json")
 
 
-def run_form(precision, steps, warmup, L=128, Ns=200):
+def run_form(precision, steps, warmup, adjoint, L=128, Ns=200):
+    import numpy as np
+
     from polystring_amd import capi, workloads
     wl = workloads.cfg2(L, Ns)
     ctx = capi.Context(0, precision)
     m = capi.ModelAB(ctx, wl.wA, wl.wB, **wl.model_kwargs())
-    m.update(warmup)
+    if adjoint:
+        m.set_adjoint(True)
+    # first update from the initial fields: the same-input precision probe
+    m.update(1)
+    first = m.observe()
+    first["phiA"] = m.get("phiA")
+    first["phiB"] = m.get("phiB")
+    if warmup > 1:
+        m.update(warmup - 1)
     ctx.synchronize()
     m.set_profiling(True)
     ctx.synchronize()
@@ -41,26 +59,29 @@ def run_form(precision, steps, warmup, L=128, Ns=200):
     t1 = time.perf_counter()
     prof = m.profile()
     obs = m.observe()
+    stress = m.last_stress().tolist() if adjoint else None
     m.set_profiling(False)
     wall = t1 - t0
     cps = m.contour_steps()
     p = 8 if precision == "fp64" else 4
-    B_step = workloads.algorithmic_bytes_per_step(wl.shape, p)
-    # dominant kernel: the propagator contour step (k_plane + k_xpass launches of
-    # one lockstep step, which advances two chains by one contour step each)
+    # dominant kernel: the propagator contour step (k_plane + k_xpass launches
+    # of one lockstep step, which advances both chains by one contour step)
+    bytes_launch = 2 * workloads.algorithmic_bytes_per_step(wl.shape, p)
+    if adjoint:
+        bytes_launch += workloads.adjoint_bytes_per_lockstep_step(wl.shape, p)
     n_launch = prof["iters"] * prof["lockstep_steps"]
     step_ms = prof["prop_ms"] / n_launch
-    achieved = 2 * B_step / (step_ms * 1e-3) / 1e9
+    achieved = bytes_launch / (step_ms * 1e-3) / 1e9
     out = dict(
-        precision=precision, wall_s=wall, iters=steps, contour_steps=cps * steps,
+        precision=precision, adjoint=adjoint, wall_s=wall, iters=steps, contour_steps=cps * steps,
         contour_steps_per_s=cps * steps / wall, iters_per_s=steps / wall,
         propagator_ms=prof["prop_ms"], density_anderson_ms=prof["rest_ms"],
-        lockstep_step_us=step_ms * 1e3, achieved_GBs=achieved, bytes_per_launch=2 * B_step,
-        H=obs["H"], residual=obs["residual"], Q=obs["Q"],
+        lockstep_step_us=step_ms * 1e3, achieved_GBs=achieved, bytes_per_launch=bytes_launch,
+        H=obs["H"], residual=obs["residual"], Q=obs["Q"], stress=stress,
     )
     del m
     ctx.close()
-    return out
+    return out, first
 
 
 def cpu_baseline(seconds_target=15.0):
@@ -104,13 +125,16 @@ def copy_probe(nbytes=1 << 30, reps=3):
 
 
 def main():
+    import numpy as np
+
     ap = argparse.ArgumentParser()
     ap.add_argument("--copy-probe", action="store_true", help="run only the traffic-calibration copy")
+    ap.add_argument("--form", choices=["both", "mixed", "fp64"], default="both",
+                    help="time only one form (for profiling runs)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-fp32", action="store_true")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
@@ -119,50 +143,67 @@ def main():
         print("bench.py runs on one GPU only: this build has no multi-GPU path (each GPU would run "
               "independent solves / string images); use --gpus 1", file=sys.stderr)
         sys.exit(2)
+    warmup = max(1, args.warmup)
 
-    f64 = run_form("fp64", args.steps, args.warmup)
-    print(f"[bench] fp64 {json.dumps(f64)}", file=sys.stderr, flush=True)
-    f32 = None
-    if not args.no_fp32:
-        f32 = run_form("fp32", args.steps, args.warmup)
-        print(f"[bench] fp32 {json.dumps(f32)}", file=sys.stderr, flush=True)
+    mixed = f64 = first32 = first64 = None
+    if args.form in ("both", "mixed"):
+        mixed, first32 = run_form("fp32", args.steps, warmup, adjoint=True)
+        print(f"[bench] mixed+adjoint {json.dumps(mixed)}", file=sys.stderr, flush=True)
+    if args.form in ("both", "fp64"):
+        f64, first64 = run_form("fp64", args.steps, warmup, adjoint=False)
+        print(f"[bench] fp64 plain {json.dumps(f64)}", file=sys.stderr, flush=True)
+    head = mixed if mixed is not None else f64
 
     traffic = None
     if os.path.exists(PROFILE_TRAFFIC):
         with open(PROFILE_TRAFFIC) as fh:
-            traffic = json.load(fh).get("hbm_bytes_per_lockstep_step_fp64")
+            key = "hbm_bytes_per_lockstep_step_" + ("fp32_adjoint" if head is mixed else "fp64")
+            traffic = json.load(fh).get(key)
     cpu = None if args.no_cpu_baseline else cpu_baseline()
 
     line = {
         "metric": "contour-steps/s at 128^3 Ns=200: mixed+adjoint vs FP64 plain, |dH/H|<=1e-6",
-        "value": f64["contour_steps_per_s"],
+        "value": head["contour_steps_per_s"],
         "unit": "contour-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": 1e3 * f64["wall_s"] / args.steps,
+        "warmup": warmup,
+        "ms_per_step": 1e3 * head["wall_s"] / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32" if head is mixed else "f64",
         "data": "synthetic (seeded double-gyroid template + Philox Gaussian noise)",
-        "config": {"workload": "AB diblock SCFT, RQM4 + Simpson + Anderson(20), 128^3, Ns=200, "
-                               "f=0.35, chiN=18, cubic a=4.2 Rg, FP64 plain",
-                   "grid": [128, 128, 128], "Ns": 200, "Ms": [71, 131], "step": "one SCFT iteration "
-                   "(2*Ns contour steps)", "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": f64["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": f64["achieved_GBs"] / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "RQM4 contour step (k_plane + k_xpass launches of one lockstep step)",
-                     "bytes_per_launch": f64["bytes_per_launch"], "avg_launch_us": f64["lockstep_step_us"]},
+        "config": {"workload": "AB diblock SCFT, RQM4 + Simpson + Anderson(20), 128^3, Ns=200, f=0.35, "
+                               "chiN=18, cubic a=4.2 Rg; " +
+                               ("FP32 propagators with FP64 Q/H/Anderson and the fused cell-stress adjoint"
+                                if head is mixed else "FP64 plain"),
+                   "grid": [128, 128, 128], "Ns": 200, "Ms": [71, 131],
+                   "step": "one SCFT iteration (2*Ns contour steps)", "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": head["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": head["achieved_GBs"] / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "RQM4 lockstep contour step (k_plane3 + k_xpass3 launches advancing both chains"
+                               + (", with the fused stress adjoint)" if head is mixed else ")"),
+                     "bytes_per_launch": head["bytes_per_launch"], "avg_launch_us": head["lockstep_step_us"]},
         "cpu_baseline": cpu,
-        "iters_per_s": f64["iters_per_s"],
-        "H_final": f64["H"],
+        "iters_per_s": head["iters_per_s"],
     }
-    if f32 is not None:
-        line["fp32_bulk"] = {"contour_steps_per_s": f32["contour_steps_per_s"], "iters_per_s": f32["iters_per_s"],
-                             "speedup_vs_fp64": f32["contour_steps_per_s"] / f64["contour_steps_per_s"],
-                             "dH_over_H_vs_fp64": abs(f32["H"] - f64["H"]) / abs(f64["H"]),
-                             "achieved_GBs": f32["achieved_GBs"], "lockstep_step_us": f32["lockstep_step_us"]}
+    if mixed is not None:
+        line["stress_dHdl"] = mixed["stress"]
+    if f64 is not None:
+        line["fp64_plain"] = {"contour_steps_per_s": f64["contour_steps_per_s"], "iters_per_s": f64["iters_per_s"],
+                              "lockstep_step_us": f64["lockstep_step_us"], "achieved_GBs": f64["achieved_GBs"],
+                              "roofline_frac": f64["achieved_GBs"] / HBM_PEAK_GBS, "H": f64["H"]}
+    if mixed is not None and f64 is not None:
+        line["speedup_vs_fp64_plain"] = mixed["contour_steps_per_s"] / f64["contour_steps_per_s"]
+        line["precision_error_same_fields"] = {
+            "what": "first update from the same initial fields, FP32-bulk vs FP64",
+            "dH_over_H": abs(first32["H"] - first64["H"]) / abs(first64["H"]),
+            "max_dphiA": float(np.abs(first32["phiA"] - first64["phiA"]).max()),
+            "max_dphiB": float(np.abs(first32["phiB"] - first64["phiB"]).max()),
+            "residual_fp32": first32["residual"], "residual_fp64": first64["residual"],
+        }
+        line["trajectory_dH_over_H_after_run"] = abs(mixed["H"] - f64["H"]) / abs(f64["H"])
     print(json.dumps(line), flush=True)
 
 

@@ -308,6 +308,10 @@ template <typename T> struct AdjointArena {
 };
 
 template <typename T> struct Program {
+    // FP64 takes exp(-w ds/2) as the square of exp(-w ds/4) (two fewer
+    // slice reads per step, a rounding-level change); FP32 keeps both tables,
+    // whose independent roundings hold the mixed-precision error of H lower.
+    static constexpr int kE2Sq = sizeof(T) == 8 ? 1 : 0;
     struct Launch {
         bool plane;
         int off, n;
@@ -370,7 +374,7 @@ template <typename T> struct Program {
                 d.qin = sg.q;
                 d.out0 = lanes[l].Sf;
                 d.pre0 = sg.e2;
-                if (rqm4) { d.out1 = lanes[l].Sh; d.pre1 = sg.e4; }
+                if (rqm4) { d.out1 = lanes[l].Sh; d.pre1 = sg.e4; d.e2sq = kE2Sq; }
                 v.push_back(d);
             }
             push_p(v);
@@ -397,6 +401,7 @@ template <typename T> struct Program {
                     x1.push_back(adjoin(sg.khalf->desc(lanes[l].Sh), *sg.khalf, t, l, 2));
                     f.in1 = lanes[l].Sh;
                     f.post1 = sg.e4;
+                    f.e2sq = kE2Sq;
                 } else {
                     x2.push_back(adjoin(sg.kfull->desc(lanes[l].Sf), *sg.kfull, t, l, 0));
                 }
@@ -766,7 +771,7 @@ template <typename T> struct Model : po_model {
             int rc = (fused_on ? prog_adj : prog).run_graph(sh, rt, st);
             if (rc) return rc;
             if (fused_on) {
-                PO_HIP(launch_sum_rows(fusedPart.p, (long)prog_adj.ndot * xdot_blocks(sh), 3, fusedSum.p, st));
+                PO_HIP(launch_sum_rows(fusedPart.p, (long)prog_adj.ndot * xdot_blocks(sh), 3, part.p, fusedSum.p, st));
                 fused_valid = true;
             } else {
                 fused_valid = false;
@@ -993,7 +998,7 @@ template <typename T> struct Model : po_model {
             if (e == hipSuccess) e = launch_xdot<T>(sh, rt, adjD.p + s0, n, st);
             if (e != hipSuccess) return fail(PO_ERR_DEVICE, std::string("adjoint launch: ") + hipGetErrorString(e));
         }
-        PO_HIP(launch_sum_rows(adjPart.p, (long)adj_N * xdot_blocks(sh), 3, adjSum.p, st));
+        PO_HIP(launch_sum_rows(adjPart.p, (long)adj_N * xdot_blocks(sh), 3, part.p, adjSum.p, st));
         return PO_OK;
     }
 

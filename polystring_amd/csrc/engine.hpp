@@ -26,6 +26,11 @@ namespace po {
 //   outj = fwd(q * prej)  for j = 0, 1                      (optional)
 // inv() is the inverse y transform followed by complex-to-real along z;
 // fwd() is real-to-complex along z followed by the forward y transform.
+//
+// With e2sq set (RQM4), post0 and pre0 are not read: the full-step factors
+// exp(-w ds/2) are taken as the squares of post1 / pre1 = exp(-w ds/4), which
+// saves two slice-sized reads per plane.  With pre0 == post0 (the mid pass)
+// the factor is read once and applied on both sides.
 template <typename T> struct PlaneDesc {
     const cx<T> *in0, *in1;
     const T *qin;
@@ -33,6 +38,7 @@ template <typename T> struct PlaneDesc {
     T *qout, *qout2;
     const T *pre0, *pre1;
     cx<T> *out0, *out1;
+    int e2sq;
 };
 
 // Per-lane descriptor of the x-pencil kernel: Sout <- invX(K * fwdX(S)) with
@@ -84,7 +90,8 @@ int xdot_blocks(const Shape &sh);
 // complex elements of one stored spectrum of the fused adjoint, -1 when the
 // grid has no full-tile x-pencil kernel
 template <typename T> long xstore_elems(const Shape &sh);  // workgroups per descriptor (partials per descriptor = 3x this)
-hipError_t launch_sum_rows(const double *partials, long nrows, int nval, double *out, hipStream_t st);
+hipError_t launch_sum_rows(const double *partials, long nrows, int nval, double *scratch, double *out,
+                           hipStream_t st);
 bool xpass_supported(int Lx);
 
 // variant 0: default workgroup size; 1: 1024 threads (planes of >= 8192 reals)

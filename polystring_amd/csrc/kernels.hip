@@ -134,21 +134,29 @@ hipError_t launch_finish_partials(const double *partials, int nblk, int nval, do
     return hipGetLastError();
 }
 
-// out[v] = sum_r partials[r * nval + v]: one workgroup per value, each
-// thread a fixed strided subset, then a fixed-order tree (deterministic).
+// out[v] = sum_r partials[r * nval + v], in two
+// fixed-order levels (deterministic): kRedBlocks workgroups each reduce a
+// contiguous chunk of rows into scratch[b * nval + v], then one pass sums the
+// workgroups in order.  scratch holds kRedBlocks * nval doubles.
 __global__ void __launch_bounds__(kRedThreads) k_sum_rows(const double *__restrict__ partials, long nrows, int nval,
-                                                          double *out) {
+                                                          double *scratch) {
     __shared__ double sh[kRedThreads / 64];
-    const int v = blockIdx.x;
-    double s = 0.0;
-    for (long r = threadIdx.x; r < nrows; r += kRedThreads) s += partials[r * nval + v];
-    s = block_sum(s, sh);
-    if (threadIdx.x == 0) out[v] = s;
+    const long chunk = (nrows + gridDim.x - 1) / gridDim.x;
+    const long r0 = blockIdx.x * chunk, r1 = r0 + chunk < nrows ? r0 + chunk : nrows;
+    for (int v = 0; v < nval; ++v) {
+        double s = 0.0;
+        for (long r = r0 + threadIdx.x; r < r1; r += kRedThreads) s += partials[r * nval + v];
+        s = block_sum(s, sh);
+        if (threadIdx.x == 0) scratch[blockIdx.x * nval + v] = s;
+    }
 }
 
-hipError_t launch_sum_rows(const double *partials, long nrows, int nval, double *out, hipStream_t st) {
-    hipLaunchKernelGGL(k_sum_rows, dim3(nval), dim3(kRedThreads), 0, st, partials, nrows, nval, out);
-    return hipGetLastError();
+hipError_t launch_sum_rows(const double *partials, long nrows, int nval, double *scratch, double *out,
+                           hipStream_t st) {
+    hipLaunchKernelGGL(k_sum_rows, dim3(kRedBlocks), dim3(kRedThreads), 0, st, partials, nrows, nval, scratch);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_finish_partials(scratch, kRedBlocks, nval, 1.0, out, st);
 }
 
 // ------------------------------------------------------------- Anderson --

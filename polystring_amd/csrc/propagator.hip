@@ -361,22 +361,38 @@ k_plane3(const PlaneDesc<T> *__restrict__ descs, Roots<T> rt, T invM) {
         }
     };
 
+    auto sq = [](cx<T> f) { return mk<T>(f.x * f.x, f.y * f.y); };
     cx<T> q[ITZ][RZ];
+    bool stored = false, premul = false;  // qout written / out0 factor applied
     if (d.in0) {
         inverse_to_regs(d.in0, q);
+        // the mid pass (in0 only, pre0 == post0) applies its factor on both sides
+        const bool both = !d.in1 && d.out0 && d.post0 && d.pre0 == d.post0 && !d.qout;
+        if (d.in1 && d.e2sq) {
 #pragma unroll
-        for (int it = 0; it < ITZ; ++it)
-            if (valid(it)) {
+            for (int it = 0; it < ITZ; ++it)
+                if (valid(it)) {
 #pragma unroll
-                for (int r = 0; r < RZ; ++r) {
-                    const cx<T> f = d.post0 ? gload(pair(d.post0, it, r)) : mk<T>((T)1, (T)1);
-                    q[it][r] = mk<T>((q[it][r].x * f.x) * invM, (q[it][r].y * f.y) * invM);
+                    for (int r = 0; r < RZ; ++r) q[it][r] = mk<T>(q[it][r].x * invM, q[it][r].y * invM);
                 }
-            }
+        } else {
+#pragma unroll
+            for (int it = 0; it < ITZ; ++it)
+                if (valid(it)) {
+#pragma unroll
+                    for (int r = 0; r < RZ; ++r) {
+                        const cx<T> f = d.post0 ? gload(pair(d.post0, it, r)) : mk<T>((T)1, (T)1);
+                        q[it][r] = mk<T>((q[it][r].x * f.x) * invM, (q[it][r].y * f.y) * invM);
+                        if (both) q[it][r] = mk<T>(f.x * q[it][r].x, f.y * q[it][r].y);
+                    }
+                }
+            premul = both;
+        }
         if (d.in1) {
-            // The full-step slice q_f waits in the output slice (the same
-            // lane reads back what it wrote; the lines are still in L2)
-            // instead of in registers across the second inverse transform.
+            // The full-step slice q_f (with e2sq: before its factor) waits in
+            // the output slice (the same lane reads back what it wrote; the
+            // lines are still in L2) instead of in registers across the
+            // second inverse transform.
 #pragma unroll
             for (int it = 0; it < ITZ; ++it)
                 if (valid(it)) {
@@ -384,6 +400,9 @@ k_plane3(const PlaneDesc<T> *__restrict__ descs, Roots<T> rt, T invM) {
                     for (int r = 0; r < RZ; ++r) gstore(const_cast<cx<T> *>(pair(d.qout, it, r)), q[it][r]);
                 }
             inverse_to_regs(d.in1, q);
+            // with e2sq and the next step in the same block (pre1 == post1),
+            // the out0 factor exp(-w ds/2) = f^2 is applied here from the f in hand
+            const bool fuse0 = d.e2sq && d.out0 && d.pre1 == d.post1;
 #pragma unroll
             for (int it = 0; it < ITZ; ++it)
                 if (valid(it)) {
@@ -391,11 +410,22 @@ k_plane3(const PlaneDesc<T> *__restrict__ descs, Roots<T> rt, T invM) {
                     for (int r = 0; r < RZ; ++r) {
                         const cx<T> f = d.post1 ? gload(pair(d.post1, it, r)) : mk<T>((T)1, (T)1);
                         const T hx = (q[it][r].x * f.x) * invM, hy = (q[it][r].y * f.y) * invM;
-                        const cx<T> qf = gload(pair(d.qout, it, r));
+                        cx<T> qf = gload(pair(d.qout, it, r));
+                        if (d.e2sq) {
+                            const cx<T> f2 = sq(f);
+                            qf = mk<T>(f2.x * qf.x, f2.y * qf.y);
+                        }
                         // src/RQM4.cc:91
                         q[it][r] = mk<T>(((T)4 * hx - qf.x) / (T)3, ((T)4 * hy - qf.y) / (T)3);
+                        if (fuse0) {
+                            gstore(const_cast<cx<T> *>(pair(d.qout, it, r)), q[it][r]);
+                            if (d.qout2) gstore(const_cast<cx<T> *>(pair(d.qout2, it, r)), q[it][r]);
+                            const cx<T> f2 = sq(f);
+                            q[it][r] = mk<T>(f2.x * q[it][r].x, f2.y * q[it][r].y);
+                        }
                     }
                 }
+            stored = premul = fuse0;
         }
     } else {
 #pragma unroll
@@ -405,7 +435,7 @@ k_plane3(const PlaneDesc<T> *__restrict__ descs, Roots<T> rt, T invM) {
                 for (int r = 0; r < RZ; ++r) q[it][r] = gload(pair(d.qin, it, r));
             }
     }
-    if (d.qout) {
+    if (d.qout && !stored) {
 #pragma unroll
         for (int it = 0; it < ITZ; ++it)
             if (valid(it)) {
@@ -417,15 +447,19 @@ k_plane3(const PlaneDesc<T> *__restrict__ descs, Roots<T> rt, T invM) {
             }
     }
     if (d.out0) {
+        if (!premul) {
 #pragma unroll
-        for (int it = 0; it < ITZ; ++it)
-            if (valid(it)) {
+            for (int it = 0; it < ITZ; ++it)
+                if (valid(it)) {
 #pragma unroll
-                for (int r = 0; r < RZ; ++r) {
-                    const cx<T> f = d.pre0 ? gload(pair(d.pre0, it, r)) : mk<T>((T)1, (T)1);
-                    q[it][r] = mk<T>(f.x * q[it][r].x, f.y * q[it][r].y);
+                    for (int r = 0; r < RZ; ++r) {
+                        cx<T> f = mk<T>((T)1, (T)1);
+                        if (d.e2sq) f = sq(gload(pair(d.pre1, it, r)));
+                        else if (d.pre0) f = gload(pair(d.pre0, it, r));
+                        q[it][r] = mk<T>(f.x * q[it][r].x, f.y * q[it][r].y);
+                    }
                 }
-            }
+        }
         forward_from_regs(q, d.out0);
     }
     if (d.out1) {
@@ -543,9 +577,11 @@ template <typename T, int LX> struct X3Cfg {
     static constexpr int NT = NT0 <= 64 ? 64 : NT0 <= 128 ? 128 : NT0 <= 256 ? 256 : 512;
     static constexpr int NW = NT / 64;
     static constexpr int CPW = C / NW;
-    static constexpr int CW = sizeof(cx<T>) / 4;
-    static constexpr int BANKMOD = 64 / CW;
-    static constexpr int RS = C + ((CPW - C % BANKMOD) % BANKMOD + BANKMOD) % BANKMOD;
+    // LDS row stride: a wave reads its CPW adjacent columns down 64 / CPW
+    // rows; with RS an odd multiple of CPW the rows of one LDS cycle start in
+    // distinct bank groups.  C / CPW = NW is a power of two, so C + CPW
+    // (or C itself for one wave) is the smallest such stride.
+    static constexpr int RS = NW == 1 ? C : C + CPW;
     static_assert(C % NW == 0, "tile columns must split evenly over waves");
 };
 
@@ -604,25 +640,28 @@ k_xpass3(const XDesc<T> *__restrict__ descs, Roots<T> rt, int ncol, int hz) {
             }
         }
     }
-    if (d.Sdot) {
-        __shared__ double red[3][X::NW];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            double a = acc[k];
-            for (int off = 32; off > 0; off >>= 1) a += __shfl_down(a, off, 64);
-            if (lane == 0) red[k][w] = a;
-        }
-        __syncthreads();
-        if (threadIdx.x < 3) {
-            double sum = 0.0;
-            for (int i = 0; i < X::NW; ++i) sum += red[threadIdx.x][i];
-            d.partials[blockIdx.x * 3 + threadIdx.x] = d.coef * sum;
-        }
-    }
     {
         cx<T> b[I::ITL][I::RL];
         WaveFFT<T, LX, true, CPW, true, IO_REG, IO_GLB>::run(lds, 1, RS, rt.x, lane, nullptr, 1, ncol,
                                                              (d.Sout ? d.Sout : d.S) + cw, v, b);
+    }
+    if (d.Sdot) {
+        // workgroup sum of the dot, through the (now free) transform buffer,
+        // so the kernel needs no LDS beyond it
+        double *red = reinterpret_cast<double *>(buf);
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            for (int off = 32; off > 0; off >>= 1) acc[k] += __shfl_down(acc[k], off, 64);
+        __syncthreads();
+        if (lane == 0)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) red[k * X::NW + w] = acc[k];
+        __syncthreads();
+        if (threadIdx.x < 3) {
+            double sum = 0.0;
+            for (int i = 0; i < X::NW; ++i) sum += red[threadIdx.x * X::NW + i];
+            d.partials[blockIdx.x * 3 + threadIdx.x] = d.coef * sum;
+        }
     }
 }
 

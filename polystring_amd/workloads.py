@@ -111,3 +111,13 @@ def algorithmic_bytes_per_step(shape, p):
     M = Lx * Ly * Lz
     Mh = Lx * Ly * (Lz // 2 + 1)
     return (13 * M + 12 * Mh) * p
+
+
+def adjoint_bytes_per_lockstep_step(shape, p):
+    """Extra bytes of the fused stress adjoint per lockstep step (two chains):
+    each chain stores three full spectra per step in the first half of the
+    contour and reads its partner's three in the second half, 3 Mh complex
+    (6 Mh p bytes) per chain-step on average."""
+    Lx, Ly, Lz = shape
+    Mh = Lx * Ly * (Lz // 2 + 1)
+    return 2 * 6 * Mh * p

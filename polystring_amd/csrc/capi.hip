@@ -448,6 +448,13 @@ template <typename T> struct Program {
     int tune(const Shape &sh, const Roots<T> &rt, hipStream_t st) {
         const int nvar = (long)sh.Ly * sh.Lz >= 8192 ? 2 : 1;
         if (nvar == 1) return PO_OK;
+        // PO_PLANE_VARIANT=0|1 pins the choice (reproducible launch shapes)
+        if (const char *env = std::getenv("PO_PLANE_VARIANT")) {
+            const int v = std::atoi(env);
+            if (v < 0 || v >= nvar) return fail(PO_ERR_INVALID, "PO_PLANE_VARIANT out of range");
+            plane_variant = v;
+            return PO_OK;
+        }
         hipEvent_t a, b;
         PO_HIP(hipEventCreate(&a));
         PO_HIP(hipEventCreate(&b));

@@ -362,8 +362,14 @@ k_plane3(const PlaneDesc<T> *__restrict__ descs, Roots<T> rt, T invM) {
     };
 
     auto sq = [](cx<T> f) { return mk<T>(f.x * f.x, f.y * f.y); };
+    // FP32 has the registers to hold a second copy of the slice: q_f across
+    // the second inverse transform, then exp(-w ds/4) q' across the out0
+    // forward transform, instead of parking / re-reading them in HBM.
+    constexpr bool kHold = sizeof(T) == 4;
     cx<T> q[ITZ][RZ];
+    cx<T> qh[kHold ? ITZ : 1][kHold ? RZ : 1];
     bool stored = false, premul = false;  // qout written / out0 factor applied
+    bool held1 = false;                   // qh holds the out1 input
     if (d.in0) {
         inverse_to_regs(d.in0, q);
         // the mid pass (in0 only, pre0 == post0) applies its factor on both sides
@@ -393,16 +399,25 @@ k_plane3(const PlaneDesc<T> *__restrict__ descs, Roots<T> rt, T invM) {
             // the output slice (the same lane reads back what it wrote; the
             // lines are still in L2) instead of in registers across the
             // second inverse transform.
+            if constexpr (kHold) {
 #pragma unroll
-            for (int it = 0; it < ITZ; ++it)
-                if (valid(it)) {
+                for (int it = 0; it < ITZ; ++it)
 #pragma unroll
-                    for (int r = 0; r < RZ; ++r) gstore(const_cast<cx<T> *>(pair(d.qout, it, r)), q[it][r]);
-                }
+                    for (int r = 0; r < RZ; ++r) qh[it][r] = q[it][r];
+            } else {
+#pragma unroll
+                for (int it = 0; it < ITZ; ++it)
+                    if (valid(it)) {
+#pragma unroll
+                        for (int r = 0; r < RZ; ++r) gstore(const_cast<cx<T> *>(pair(d.qout, it, r)), q[it][r]);
+                    }
+            }
             inverse_to_regs(d.in1, q);
             // with e2sq and the next step in the same block (pre1 == post1),
             // the out0 factor exp(-w ds/2) = f^2 is applied here from the f in hand
             const bool fuse0 = d.e2sq && d.out0 && d.pre1 == d.post1;
+            // with the next step in the same block, the out1 input f q' is formed here
+            held1 = kHold && d.out1 && d.pre1 == d.post1;
 #pragma unroll
             for (int it = 0; it < ITZ; ++it)
                 if (valid(it)) {
@@ -410,13 +425,17 @@ k_plane3(const PlaneDesc<T> *__restrict__ descs, Roots<T> rt, T invM) {
                     for (int r = 0; r < RZ; ++r) {
                         const cx<T> f = d.post1 ? gload(pair(d.post1, it, r)) : mk<T>((T)1, (T)1);
                         const T hx = (q[it][r].x * f.x) * invM, hy = (q[it][r].y * f.y) * invM;
-                        cx<T> qf = gload(pair(d.qout, it, r));
+                        cx<T> qf;
+                        if constexpr (kHold) qf = qh[it][r];
+                        else qf = gload(pair(d.qout, it, r));
                         if (d.e2sq) {
                             const cx<T> f2 = sq(f);
                             qf = mk<T>(f2.x * qf.x, f2.y * qf.y);
                         }
                         // src/RQM4.cc:91
                         q[it][r] = mk<T>(((T)4 * hx - qf.x) / (T)3, ((T)4 * hy - qf.y) / (T)3);
+                        if constexpr (kHold)
+                            if (held1) qh[it][r] = mk<T>(f.x * q[it][r].x, f.y * q[it][r].y);
                         if (fuse0) {
                             gstore(const_cast<cx<T> *>(pair(d.qout, it, r)), q[it][r]);
                             if (d.qout2) gstore(const_cast<cx<T> *>(pair(d.qout2, it, r)), q[it][r]);
@@ -463,19 +482,27 @@ k_plane3(const PlaneDesc<T> *__restrict__ descs, Roots<T> rt, T invM) {
         forward_from_regs(q, d.out0);
     }
     if (d.out1) {
-        // the slice is re-read from where this lane stored it (qout, or the
-        // input slice) rather than kept in registers across out0
-        const T *qs = d.qout ? d.qout : d.qin;
+        if (held1) {
+            if constexpr (kHold)
 #pragma unroll
-        for (int it = 0; it < ITZ; ++it)
-            if (valid(it)) {
+                for (int it = 0; it < ITZ; ++it)
 #pragma unroll
-                for (int r = 0; r < RZ; ++r) {
-                    const cx<T> f = d.pre1 ? gload(pair(d.pre1, it, r)) : mk<T>((T)1, (T)1);
-                    const cx<T> v = gload(pair(qs, it, r));
-                    q[it][r] = mk<T>(f.x * v.x, f.y * v.y);
+                    for (int r = 0; r < RZ; ++r) q[it][r] = qh[it][r];
+        } else {
+            // the slice is re-read from where this lane stored it (qout, or
+            // the input slice) rather than kept in registers across out0
+            const T *qs = d.qout ? d.qout : d.qin;
+#pragma unroll
+            for (int it = 0; it < ITZ; ++it)
+                if (valid(it)) {
+#pragma unroll
+                    for (int r = 0; r < RZ; ++r) {
+                        const cx<T> f = d.pre1 ? gload(pair(d.pre1, it, r)) : mk<T>((T)1, (T)1);
+                        const cx<T> v = gload(pair(qs, it, r));
+                        q[it][r] = mk<T>(f.x * v.x, f.y * v.y);
+                    }
                 }
-            }
+        }
         forward_from_regs(q, d.out1);
     }
 }

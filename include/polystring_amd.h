@@ -144,6 +144,17 @@ long po_model_contour_steps(const po_model *m);
  * separable k tables: RQM4 on any cell (it uses the orthogonal k^2,
  * src/RQM4.cc:9-16), OS2 on orthogonal cells. */
 int po_model_stress(po_model *m, double *dHdl3, double *Q);
+/* Field gradient by the adjoint of the chain: runs the four propagators with
+ * the current fields, then returns gA[r] = M d(ln Q)/dwA(r) and gB likewise
+ * (C order, M = Lx Ly Lz), exact for the discrete RQM4 / OS2 chain.  To the
+ * contour discretisation these equal -phiA(r), -phiB(r) (the densities of
+ * src/Model_AB.cc:195-199); the -ln Q term of H contributes -g to dH/dw.
+ * No reference counterpart.  Q (optional) gets Q. */
+int po_model_grad_w(po_model *m, double *gA, double *gB, double *Q);
+/* Replace the fields (host, C order), as a File init does with the current
+ * cell: Boltzmann factors rebuilt, Anderson history restarted from them
+ * (src/Model_AB.cc:633-644, src/FieldAX.cc:34-39). */
+int po_model_set_fields(po_model *m, const double *wA, const double *wB);
 /* Fused adjoint: with on != 0 every later po_model_update also computes the
  * cell stress of the fields it solved with, inside the propagator passes
  * (spectra of the first half of the chains are kept and dotted with the

@@ -32,7 +32,8 @@ EXPORTS = [
     "po_prop_set_head_from_tail", "po_prop_solve", "po_prop_get_slice", "po_prop_Q", "po_density",
     "po_model_create", "po_model_destroy", "po_model_update", "po_model_observe", "po_model_get",
     "po_model_relax", "po_model_contour_steps", "po_model_stress", "po_model_set_cell",
-    "po_model_relax_cell", "po_model_set_adjoint", "po_model_last_stress",
+    "po_model_relax_cell", "po_model_set_adjoint", "po_model_last_stress", "po_model_grad_w",
+    "po_model_set_fields",
     "po_model_set_profiling", "po_model_profile",
 ]
 
@@ -87,6 +88,7 @@ def load():
         "po_model_stress": ([_vp, _dp, _dp], i), "po_model_set_cell": ([_vp] + [d] * 6, i),
         "po_model_relax_cell": ([_vp, i, d, i, i, i, d, _dp, _dp, _dp, ctypes.POINTER(i)], i),
         "po_model_set_adjoint": ([_vp, i], i), "po_model_last_stress": ([_vp, _dp], i),
+        "po_model_grad_w": ([_vp, _dp, _dp, _dp], i), "po_model_set_fields": ([_vp, _dp, _dp], i),
         "po_model_set_profiling": ([_vp, i], i), "po_model_profile": ([_vp, _dp], i),
     }
     for name, (args, res) in sig.items():
@@ -260,6 +262,19 @@ class ModelAB:
                                           int(record_interval), float(thresh), _p(cell), _p(H), _p(st),
                                           ctypes.byref(k)))
         return cell, float(H[0]), st, k.value
+
+    def grad_w(self):
+        """(M dlnQ/dwA, M dlnQ/dwB, Q) at the current fields; ~ (-phiA, -phiB)."""
+        gA = np.zeros(self.shape)
+        gB = np.zeros(self.shape)
+        q = np.zeros(1)
+        _check(load().po_model_grad_w(self.h, _p(gA), _p(gB), _p(q)))
+        return gA, gB, float(q[0])
+
+    def set_fields(self, wA, wB):
+        wA = np.ascontiguousarray(wA, dtype=np.float64).reshape(self.shape)
+        wB = np.ascontiguousarray(wB, dtype=np.float64).reshape(self.shape)
+        _check(load().po_model_set_fields(self.h, _p(wA), _p(wB)))
 
     def set_adjoint(self, on=True):
         """Fused per-update stress (see last_stress)."""

@@ -647,6 +647,8 @@ struct po_model {
     virtual int get(int which, double *out) = 0;
     virtual int stress(double *dHdl, double *Q) = 0;
     virtual int set_adjoint(bool on) = 0;
+    virtual int grad_w(double *gA, double *gB, double *Q) = 0;
+    virtual int set_fields(const double *wA, const double *wB) = 0;
     virtual int last_stress(double *dHdl) = 0;
     virtual int set_cell(const double *lengths_angles) = 0;
 };
@@ -1030,6 +1032,167 @@ template <typename T> struct Model : po_model {
         return PO_OK;
     }
 
+    // ------------------------------------------------- field gradient --
+    // M d(ln Q)/dw_K(r) for K = A, B by the adjoint of the chain (see
+    // k_grad_accum): RQM4 needs S_f and S_h of both slices of every step,
+    // formed in batches of g_nb steps (4 spectra + 4 slices per step).
+    DBuf<cx<T>> gS;
+    DBuf<T> gR;
+    DBuf<double> gG;
+    DBuf<PlaneDesc<T>> gP1, gP2;
+    DBuf<XDesc<T>> gX;
+    DBuf<GradStep<T>> gSt;
+    int g_nb = 0;
+    bool g_ready = false;
+
+    int grad_build() {
+        if (g_ready) return PO_OK;
+        const bool rqm4 = desc.algo == PO_RQM4;
+        hipStream_t st = ctx->stream;
+        const int nA = sA - 1, nB = sB - 1, N = nA + nB;
+        const long M = sh.M;
+        const size_t per_step = (size_t)2 * (2 * sh.Mh * sizeof(cx<T>) + 2 * M * sizeof(T));
+        int nb = N;
+        if (rqm4) {
+            nb = (int)std::max<size_t>(1, ((size_t)1 << 30) / per_step);
+            if (nb > N) nb = N;
+            PO_HIP(gS.alloc((size_t)nb * 4 * sh.Mh));
+            PO_HIP(gR.alloc((size_t)nb * 4 * M));
+        }
+        g_nb = nb;
+        PO_HIP(gG.alloc((size_t)2 * M));
+        PO_HIP(qscal.alloc(1));
+        std::vector<PlaneDesc<T>> P1(rqm4 ? 2 * N : 0), P2(rqm4 ? 4 * N : 0);
+        std::vector<XDesc<T>> X(rqm4 ? 4 * N : 0);
+        std::vector<GradStep<T>> G(N);
+        for (int s = 0; s < N; ++s) {
+            const bool inA = s < nA;
+            const int t = N - 1 - s;
+            T *q[2] = {inA ? qA.p + (size_t)s * M : qB.p + (size_t)(s - nA) * M,
+                       t < nB ? qBc.p + (size_t)t * M : qAc.p + (size_t)(t - nB) * M};
+            GradStep<T> g{};
+            g.u = q[0];
+            g.up = q[0] + M;  // the next slice of the same propagator (s + 1 <= its length - 1)
+            g.v = q[1];
+            g.vp = q[1] + M;
+            if (rqm4) {
+                const T *e2 = inA ? eA2.p : eB2.p, *e4 = inA ? eA4.p : eB4.p;
+                const KTable<T> &kf = inA ? kA : kB, &kh = inA ? kAh : kBh;
+                const int j = s % nb;
+                T *R[2][2];
+                for (int l = 0; l < 2; ++l) {
+                    cx<T> *U2 = gS.p + ((size_t)(j * 2 + l) * 2 + 0) * sh.Mh, *U4 = U2 + sh.Mh;
+                    R[l][0] = gR.p + ((size_t)(j * 2 + l) * 2 + 0) * M;
+                    R[l][1] = R[l][0] + M;
+                    PlaneDesc<T> d{};
+                    d.qin = q[l];
+                    d.out0 = U2;
+                    d.pre0 = e2;
+                    d.out1 = U4;
+                    d.pre1 = e4;
+                    P1[2 * s + l] = d;
+                    X[4 * s + 2 * l] = kf.desc(U2);
+                    X[4 * s + 2 * l + 1] = kh.desc(U4);
+                    PlaneDesc<T> f{};
+                    f.in0 = U2;
+                    f.post0 = e2;
+                    f.qout = R[l][0];
+                    PlaneDesc<T> h{};
+                    h.in0 = U4;
+                    h.post0 = e4;
+                    h.qout = R[l][1];
+                    P2[4 * s + 2 * l] = f;
+                    P2[4 * s + 2 * l + 1] = h;
+                }
+                g.Sfu = R[0][0];
+                g.Sfv = R[1][0];
+                g.Shu = R[0][1];
+                g.Shv = R[1][1];
+            }
+            G[s] = g;
+        }
+        if (rqm4) {
+            PO_HIP(gP1.alloc(P1.size()));
+            PO_HIP(gP2.alloc(P2.size()));
+            PO_HIP(gX.alloc(X.size()));
+            PO_HIP(hipMemcpyAsync(gP1.p, P1.data(), sizeof(PlaneDesc<T>) * P1.size(), hipMemcpyHostToDevice, st));
+            PO_HIP(hipMemcpyAsync(gP2.p, P2.data(), sizeof(PlaneDesc<T>) * P2.size(), hipMemcpyHostToDevice, st));
+            PO_HIP(hipMemcpyAsync(gX.p, X.data(), sizeof(XDesc<T>) * X.size(), hipMemcpyHostToDevice, st));
+        }
+        PO_HIP(gSt.alloc(G.size()));
+        PO_HIP(hipMemcpyAsync(gSt.p, G.data(), sizeof(GradStep<T>) * G.size(), hipMemcpyHostToDevice, st));
+        PO_HIP(hipStreamSynchronize(st));
+        g_ready = true;
+        return PO_OK;
+    }
+
+    int grad_w(double *gAh, double *gBh, double *Qout) override {
+        int rc;
+        if ((rc = grad_build())) return rc;
+        hipStream_t st = ctx->stream;
+        const bool rqm4 = desc.algo == PO_RQM4;
+        const long M = sh.M;
+        const int nA = sA - 1, N = (sA - 1) + (sB - 1);
+        if ((rc = prog.run(sh, rt, st))) return rc;
+        PO_HIP(launch_mean_partials<T>(qB.p + (size_t)(sB - 1) * M, M, part.p, st));
+        PO_HIP(launch_finish_partials(part.p, kRedBlocks, 1, (double)M, adjSumQ(), st));
+        PO_HIP(launch_fill<double>(gG.p, 0.0, 2 * M, st));
+        const int pv = prog.plane_variant;
+        for (int s0 = 0; s0 < N;) {
+            const int end = s0 < nA ? nA : N;
+            const int n = std::min(g_nb, end - s0);
+            hipError_t e = hipSuccess;
+            if (rqm4) {
+                e = launch_plane<T>(sh, rt, gP1.p + 2 * s0, 2 * n, st, pv);
+                if (e == hipSuccess) e = launch_xpass<T>(sh, rt, gX.p + 4 * s0, 4 * n, st);
+                if (e == hipSuccess) e = launch_plane<T>(sh, rt, gP2.p + 4 * s0, 4 * n, st, pv);
+            }
+            if (e == hipSuccess)
+                e = launch_grad_accum<T>(gSt.p + s0, n, s0 < nA ? dsA : dsB, gG.p + (s0 < nA ? 0 : M), M, st);
+            if (e != hipSuccess) return fail(PO_ERR_DEVICE, std::string("gradient launch: ") + hipGetErrorString(e));
+            s0 += n;
+        }
+        double Q = 0.0;
+        PO_HIP(hipMemcpyAsync(&Q, adjSumQ(), sizeof(double), hipMemcpyDeviceToHost, st));
+        PO_HIP(hipMemcpyAsync(gAh, gG.p, sizeof(double) * M, hipMemcpyDeviceToHost, st));
+        PO_HIP(hipMemcpyAsync(gBh, gG.p + M, sizeof(double) * M, hipMemcpyDeviceToHost, st));
+        PO_HIP(hipStreamSynchronize(st));
+        for (long i = 0; i < M; ++i) {
+            gAh[i] /= Q;
+            gBh[i] /= Q;
+        }
+        if (Qout) *Qout = Q;
+        return PO_OK;
+    }
+
+    DBuf<double> qscal;  // Q of the gradient pass
+    double *adjSumQ() { return qscal.p; }
+
+    // New fields (as a File init with the current cell): Boltzmann factors
+    // rebuilt, Anderson history restarted from them (src/FieldAX.cc:34-39).
+    int set_fields(const double *hA, const double *hB) override {
+        hipStream_t st = ctx->stream;
+        const long M = sh.M;
+        PO_HIP(hipStreamSynchronize(st));
+        PO_HIP(hipMemcpyAsync(wA.p, hA, sizeof(double) * M, hipMemcpyHostToDevice, st));
+        PO_HIP(hipMemcpyAsync(wB.p, hB, sizeof(double) * M, hipMemcpyHostToDevice, st));
+        PO_HIP(launch_exp_factor<T>(wA.p, eA2.p, -0.5 * dsA, M, st));
+        PO_HIP(launch_exp_factor<T>(wA.p, eA4.p, -0.25 * dsA, M, st));
+        PO_HIP(launch_exp_factor<T>(wB.p, eB2.p, -0.5 * dsB, M, st));
+        PO_HIP(launch_exp_factor<T>(wB.p, eB4.p, -0.25 * dsB, M, st));
+        return restart_history();
+    }
+
+    int restart_history() {
+        hipStream_t st = ctx->stream;
+        cur = n;
+        k = 0;
+        PO_HIP(hipMemcpyAsync(wAh.p + (size_t)n * sh.M, wA.p, sizeof(double) * sh.M, hipMemcpyDeviceToDevice, st));
+        PO_HIP(hipMemcpyAsync(wBh.p + (size_t)n * sh.M, wB.p, sizeof(double) * sh.M, hipMemcpyDeviceToDevice, st));
+        PO_HIP(hipStreamSynchronize(st));
+        return PO_OK;
+    }
+
     // New cell lengths / angles (same crystal system and grid): refills the
     // K tables in place, so the chain programs keep their descriptors, and
     // restarts the Anderson history from the current fields, as the
@@ -1059,12 +1222,7 @@ template <typename T> struct Model : po_model {
             if ((rc = build_ktable<T>(kAh, sh, cell, false, 0.5 * dsA, st))) return rc;
             if ((rc = build_ktable<T>(kBh, sh, cell, false, 0.5 * dsB, st))) return rc;
         }
-        cur = n;
-        k = 0;
-        PO_HIP(hipMemcpyAsync(wAh.p + (size_t)n * sh.M, wA.p, sizeof(double) * sh.M, hipMemcpyDeviceToDevice, st));
-        PO_HIP(hipMemcpyAsync(wBh.p + (size_t)n * sh.M, wB.p, sizeof(double) * sh.M, hipMemcpyDeviceToDevice, st));
-        PO_HIP(hipStreamSynchronize(st));
-        return PO_OK;
+        return restart_history();
     }
 
     int get(int which, double *out) override {
@@ -1376,6 +1534,18 @@ int po_model_relax_cell(po_model *m, int max_outer, double stress_tol, int max_i
     if (stress3) for (int p = 0; p < 3; ++p) stress3[p] = p < np ? gp[p] : 0.0;
     if (outer) *outer = std::min(k + 1, max_outer);
     return PO_OK;
+}
+
+int po_model_grad_w(po_model *m, double *gA, double *gB, double *Q) {
+    if (!m || !gA || !gB) return fail(PO_ERR_INVALID, "null argument");
+    PO_HIP(hipSetDevice(m->ctx->device));
+    return m->grad_w(gA, gB, Q);
+}
+
+int po_model_set_fields(po_model *m, const double *wA, const double *wB) {
+    if (!m || !wA || !wB) return fail(PO_ERR_INVALID, "null argument");
+    PO_HIP(hipSetDevice(m->ctx->device));
+    return m->set_fields(wA, wB);
 }
 
 int po_model_set_adjoint(po_model *m, int on) {

@@ -86,6 +86,16 @@ template <typename T> struct Roots {
 bool plane_supported(int Ly, int Lz);
 template <typename T>
 hipError_t launch_xdot(const Shape &sh, const Roots<T> &rt, const DotDesc<T> *d_desc, int ndesc, hipStream_t st);
+// One contour step of the field-gradient accumulation (see k_grad_accum):
+// u = q_s, up = q_{s+1} (forward chain), v = qc_{N-1-s}, vp = qc_{N-s}
+// (complementary chain) and, for RQM4, the full- and half-step images
+// S_f u, S_f v, S_h u, S_h v.
+template <typename T> struct GradStep {
+    const T *u, *up, *v, *vp;
+    const T *Sfu, *Sfv, *Shu, *Shv;
+};
+template <typename T>
+hipError_t launch_grad_accum(const GradStep<T> *steps, int n, double ds, double *G, long M, hipStream_t st);
 int xdot_blocks(const Shape &sh);
 // complex elements of one stored spectrum of the fused adjoint, -1 when the
 // grid has no full-tile x-pencil kernel

@@ -184,3 +184,59 @@ def test_fused_adjoint_rejects_partial_tiles(ctx64):
         m.set_adjoint(True)
     with pytest.raises(capi.POError):
         m.last_stress()
+
+
+@pytest.mark.parametrize("algo", ["RQM4", "OS2"])
+def test_grad_w_matches_finite_differences(ctx64, algo):
+    """M dlnQ/dw at single grid points against central differences of the oracle's ln Q."""
+    shape, Ms = (16, 16, 16), (13, 19)
+    lengths = (3.8, 4.0, 4.3)
+    wA = smooth_field(shape, 71, offset=2.0)
+    wB = smooth_field(shape, 72, offset=-1.0)
+    m = capi.ModelAB(ctx64, wA, wB, cell=("ORTHORHOMBIC",) + lengths, Ms=Ms, chiN=15.0, algo=algo, n_anderson=3)
+    gA, gB, Q = m.grad_w()
+    M = wA.size
+    h = 1e-5
+    for name, g, k in (("A", gA, 0), ("B", gB, 1)):
+        for idx in [(0, 0, 0), (5, 9, 3), (15, 2, 11)]:
+            wp = [wA.copy(), wB.copy()]
+            wm = [wA.copy(), wB.copy()]
+            wp[k][idx] += h
+            wm[k][idx] -= h
+            fd = M * (oracle_lnQ(*wp, 3, "ORTHORHOMBIC", lengths, Ms, algo)
+                      - oracle_lnQ(*wm, 3, "ORTHORHOMBIC", lengths, Ms, algo)) / (2 * h)
+            print(f"  {algo} {name}{idx}: adjoint {g[idx]!r} fd {fd!r}", flush=True)
+            assert abs(g[idx] - fd) <= 1e-6 * abs(fd)
+
+
+def test_grad_w_approaches_minus_phi(ctx64):
+    """M dlnQ/dw = -phi up to the contour discretisation: the gap shrinks at
+    least quadratically when ds halves."""
+    shape, lengths = (16, 16, 16), (3.8, 4.0, 4.3)
+    wA = smooth_field(shape, 81, offset=2.0)
+    wB = smooth_field(shape, 82, offset=-1.0)
+    gaps = []
+    for Ms in ((13, 29), (25, 57)):
+        m = capi.ModelAB(ctx64, wA, wB, cell=("ORTHORHOMBIC",) + lengths, Ms=Ms, chiN=15.0, n_anderson=3)
+        gA, gB, _ = m.grad_w()
+        m.update(1)  # densities of the same (initial) fields
+        gap = max(np.abs(gA + m.get("phiA")).max(), np.abs(gB + m.get("phiB")).max())
+        gaps.append(gap)
+    print(f"\n  |g + phi| at ds and ds/2: {gaps}", flush=True)
+    assert gaps[1] < 1e-3
+    assert gaps[0] / gaps[1] > 3.5
+
+
+def test_torch_autograd_binding(ctx64):
+    torch = pytest.importorskip("torch")
+    from polystring_amd.autograd import log_q
+    shape, Ms = (8, 8, 8), (5, 7)
+    wA = torch.tensor(smooth_field(shape, 91, offset=2.0), dtype=torch.float64, requires_grad=True)
+    wB = torch.tensor(smooth_field(shape, 92, offset=-1.0), dtype=torch.float64, requires_grad=True)
+    m = capi.ModelAB(ctx64, wA.detach().numpy(), wB.detach().numpy(), cell=("CUBIC", 3.5), Ms=Ms, chiN=12.0,
+                     n_anderson=2)
+    assert torch.autograd.gradcheck(lambda a, b: log_q(a, b, m), (wA, wB), eps=1e-6, atol=1e-9, rtol=1e-6)
+    lnq = log_q(wA, wB, m)
+    lnq.backward()
+    ref = oracle_lnQ(wA.detach().numpy(), wB.detach().numpy(), 3, "CUBIC", (3.5, 0.0, 0.0), Ms, "RQM4")
+    assert abs(lnq.item() - ref) <= 1e-12 * abs(ref)

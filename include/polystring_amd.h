@@ -181,6 +181,44 @@ int po_model_set_cell(po_model *m, double a, double b, double c, double alpha, d
 int po_model_relax_cell(po_model *m, int max_outer, double stress_tol, int max_iter, int min_iter,
                         int record_interval, double thresh, double *cell6, double *H, double *stress3, int *outer);
 
+/* ---------------------------------------------------------- string method */
+/* The string driver scft/polystring_AB.cc with every image on the GPU at once:
+ * m images of one AB melt, each iterated by Model_AB's simple mixing (EM)
+ * branch (src/Model_AB.cc:136-261), its propagator chains advanced as lanes
+ * of one batched launch program.  wA, wB passed to po_string_create hold the
+ * m images back to back (m * Lx*Ly*Lz doubles, C order); images 0 and m-1
+ * are the pinned ends (scft/polystring_AB.cc:236-255). */
+typedef struct po_string po_string;
+typedef struct {
+    po_grid_desc grid;
+    int sA, sB;          /* Ms */
+    double chiN;
+    double lamA, lamB;   /* lam[0], lam[1]: field mixing */
+    double lamY;         /* lam[2]: yita mixing, or the compressibility penalty */
+    int algo;            /* PO_RQM4 or PO_OS2 */
+    int contour;         /* PO_QUAD_AUTO (Simpson / Quad4_Closed) or PO_QUAD_TRAP */
+    int compressible;    /* is_compressible (src/Model_AB.cc:205-229) */
+    int mirror;          /* confine_mold = NBC_by_PBC (src/Model_AB.cc:148-182) */
+    int m;               /* num_of_beads, 3..64 */
+} po_string_desc;
+
+int po_string_create(po_ctx *ctx, const po_string_desc *desc, const double *wA, const double *wB,
+                     po_string **out);
+int po_string_destroy(po_string *s);
+/* One string iteration (scft/polystring_AB.cc:130-146): every image relaxed
+ * from its fields with a fresh yita for up to max_iter EM updates, with the
+ * scft::relax stopping rules per image (src/scft.cc:382-422,536-568); H[m]
+ * gets each image's H; then the normalised arc length of wA along the string
+ * (S[m], optional, :283-297) and the natural cubic-spline redistribution of
+ * wA and wB to equal arc length with the end images restored (:299-329). */
+int po_string_iterate(po_string *s, int max_iter, int min_iter, int record_interval, double thresh, double *H,
+                      double *S);
+/* The m images of wA, wB (after the redistribution) or of phiA, phiB (of the
+ * last EM update of each image). */
+int po_string_get(po_string *s, int which, double *host_out);
+/* contour steps of one EM update of all images (2 m Ns) */
+long po_string_contour_steps(const po_string *s);
+
 /* Measurement hooks; no counterpart in the reference (which records CPU
  * clock() per iteration, src/scft.cc:398-412).  With profiling on, every
  * update() records three HIP events on the model's stream: before the

@@ -60,6 +60,13 @@ def _load():
     lib.or_model_relax.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                    ctypes.c_double, _dp, _dp]
     lib.or_model_relax.restype = ctypes.c_int
+    lib.or_em_create.argtypes = ([ctypes.c_int] * 4 + [ctypes.c_int] + [ctypes.c_double] * 6
+                                 + [ctypes.c_int, ctypes.c_int] + [ctypes.c_double] * 4
+                                 + [ctypes.c_int] * 4 + [_dp, _dp])
+    lib.or_em_create.restype = ctypes.c_void_p
+    lib.or_model_set_fields.argtypes = [ctypes.c_void_p, _dp, _dp]
+    lib.or_string_arclength.argtypes = [ctypes.c_int, ctypes.c_long, _dp, _dp]
+    lib.or_spline_weights.argtypes = [ctypes.c_int, _dp, _dp]
     lib.or_num_threads.restype = ctypes.c_int
     lib.or_set_num_threads.argtypes = [ctypes.c_int]
     return lib
@@ -206,6 +213,74 @@ class Model:
         r = np.zeros(1)
         it = lib().or_model_relax(self._h, max_iter, min_iter, record_interval, thresh, _p(H), _p(r))
         return it, float(H[0]), float(r[0])
+
+
+class EMModel(Model):
+    """Model_AB with simple mixing (the string method's iteration) on the oracle."""
+
+    def __init__(self, wA0, wB0, *, dim, cell, Ms, chiN, lam=(0.1, 0.1, 1.0), mirror=False,
+                 compressible=False, mde="RQM4", contour="SIMPSON"):
+        wA0 = np.ascontiguousarray(wA0, dtype=np.float64)
+        wB0 = np.ascontiguousarray(wB0, dtype=np.float64)
+        self.shape = wA0.shape
+        cs, a, b, c, al, be, ga = (list(cell) + [0.0] * 7)[:7]
+        code = CRYSTAL[cs] if isinstance(cs, str) else int(cs)
+        self._h = lib().or_em_create(dim, *self.shape, code, a, b, c, al, be, ga, int(Ms[0]), int(Ms[1]),
+                                     chiN, lam[0], lam[1], lam[2], int(bool(mirror)), int(bool(compressible)),
+                                     0 if mde.upper() == "RQM4" else 1, 0 if contour.upper() == "SIMPSON" else 1,
+                                     _p(wA0), _p(wB0))
+        if not self._h:
+            raise ValueError("oracle model creation failed")
+
+    def set_fields(self, wA, wB):
+        lib().or_model_set_fields(self._h, _p(np.ascontiguousarray(wA, dtype=np.float64)),
+                                  _p(np.ascontiguousarray(wB, dtype=np.float64)))
+
+
+def string_arclength(wa):
+    """wa: (m, ...) image fields; normalised arc length S (m,)."""
+    wa = np.ascontiguousarray(wa, dtype=np.float64)
+    m = wa.shape[0]
+    S = np.zeros(m)
+    lib().or_string_arclength(m, wa[0].size, _p(wa), _p(S))
+    return S
+
+
+def spline_weights(S):
+    S = np.ascontiguousarray(S, dtype=np.float64)
+    m = S.size
+    W = np.zeros((m, m))
+    lib().or_spline_weights(m, _p(S), _p(W))
+    return W
+
+
+def string_iterate(models, wa, wb, ends, relax):
+    """One iteration of the string driver (scft/polystring_AB.cc:130-146,
+    256-329): relax every image from its fields (models[s] reset to them),
+    record H, take the arc length of wa, respline wa and wb at equal arc
+    length, then restore the pinned end images.  relax = (max_iter, min_iter,
+    record_interval, thresh).  Returns (H, S, wa, wb, phia, phib)."""
+    m = len(models)
+    H = np.zeros(m)
+    phia = np.zeros_like(wa)
+    phib = np.zeros_like(wb)
+    wa = wa.copy()
+    wb = wb.copy()
+    for s in range(m):
+        models[s].set_fields(wa[s], wb[s])
+        models[s].relax(*relax)
+        H[s] = models[s].H()  # spmodel->H() after the run (scft/polystring_AB.cc:267)
+        wa[s] = models[s].get("wA")
+        wb[s] = models[s].get("wB")
+        phia[s] = models[s].get("phiA")
+        phib[s] = models[s].get("phiB")
+    S = string_arclength(wa)
+    W = spline_weights(S)
+    shape = wa.shape
+    wa = (W @ wa.reshape(m, -1)).reshape(shape)
+    wb = (W @ wb.reshape(m, -1)).reshape(shape)
+    wa[0], wb[0], wa[-1], wb[-1] = ends
+    return H, S, wa, wb, phia, phib
 
 
 def set_threads(n):

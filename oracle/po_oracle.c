@@ -595,6 +595,10 @@ struct or_model {
     double *phiA, *phiB;
     double *tmp, *tmp2, *W, *D;
     fieldax wA, wB;
+    /* simple-mixing (EM) branch, the string method's iteration */
+    int em, mirror, compressible;
+    double lamA, lamB, lamY;
+    double *yita;
 };
 
 or_model *or_model_create(int dim, int Lx, int Ly, int Lz,
@@ -646,6 +650,7 @@ or_model *or_model_create(int dim, int Lx, int Ly, int Lz,
 
 void or_model_destroy(or_model *m) {
     if (!m) return;
+    free(m->yita);
     free(m->k2neg); free(m->lapA); free(m->lapB);
     free(m->qA); free(m->qB); free(m->qAc); free(m->qBc);
     free(m->phiA); free(m->phiB); free(m->tmp); free(m->tmp2); free(m->W); free(m->D);
@@ -667,7 +672,69 @@ static void density(or_model *m, int len, const double *q, const double *qc,
     else or_quad4_closed(m->M, len, q, qc, cc, ds, out);
 }
 
+/* NBC by PBC (src/Model_AB.cc:148-182): the upper half of the last
+ * dimension's axis (x in 1-D, y in 2-D, z in 3-D) mirrors the lower half. */
+static void mirror_field(const or_model *m, double *w) {
+    int Lx = m->Lx, Ly = m->Ly, Lz = m->Lz;
+    for (int i = 0; i < Lx; i++)
+        for (int j = 0; j < Ly; j++)
+            for (int k = 0; k < Lz; k++) {
+                int si = i, sj = j, sk = k;
+                if (m->dim == 1) { if (i < Lx / 2) continue; si = Lx - 1 - i; }
+                else if (m->dim == 2) { if (j < Ly / 2) continue; sj = Ly - 1 - j; }
+                else { if (k < Lz / 2) continue; sk = Lz - 1 - k; }
+                w[((long)i * Ly + j) * Lz + k] = w[((long)si * Ly + sj) * Lz + sk];
+            }
+}
+
+static void em_update(or_model *m) {
+    long M = m->M;
+    double *wA = m->wA.data, *wB = m->wB.data;
+    if (m->mirror) { mirror_field(m, wA); mirror_field(m, wB); }
+    solve_prop(m, m->qA, m->sA, m->dsA, m->lapA, wA);
+    memcpy(m->qB, m->qA + (long)(m->sA - 1) * M, sizeof(double) * M);
+    solve_prop(m, m->qB, m->sB, m->dsB, m->lapB, wB);
+    solve_prop(m, m->qBc, m->sB, m->dsB, m->lapB, wB);
+    memcpy(m->qAc, m->qBc + (long)(m->sB - 1) * M, sizeof(double) * M);
+    solve_prop(m, m->qAc, m->sA, m->dsA, m->lapA, wA);
+    m->Q = or_mean((int)M, m->qB + (long)(m->sB - 1) * M);
+    density(m, m->sA, m->qA, m->qAc, 1.0 / m->Q, m->dsA, m->phiA);
+    density(m, m->sB, m->qB, m->qBc, 1.0 / m->Q, m->dsB, m->phiB);
+    if (m->compressible) {
+        /* src/Model_AB.cc:205-206,225-226: eta = lamYita (phiA + phiB - 1) */
+        for (long i = 0; i < M; i++) {
+            double eta = m->lamY * (m->phiA[i] + m->phiB[i] - 1.0);
+            wA[i] = wA[i] * (1.0 - m->lamA) + m->lamA * (m->chiN * m->phiB[i] + eta);
+            wB[i] = wB[i] * (1.0 - m->lamB) + m->lamB * (m->chiN * m->phiA[i] + eta);
+        }
+        return;
+    }
+    /* src/Model_AB.cc:254-258: yita first, then the fields (Yita::update,
+     * Field::update, src/Yita.cc:49-51, src/Field.cc:51-54) */
+    for (long i = 0; i < M; i++) {
+        m->yita[i] += m->lamY * (m->phiA[i] + m->phiB[i] - 1.0);
+        wA[i] = wA[i] * (1.0 - m->lamA) + m->lamA * (m->chiN * m->phiB[i] + m->yita[i]);
+        wB[i] = wB[i] * (1.0 - m->lamB) + m->lamB * (m->chiN * m->phiA[i] + m->yita[i]);
+    }
+}
+
+or_model *or_em_create(int dim, int Lx, int Ly, int Lz, int cs, double a, double b, double c,
+                       double alpha, double beta, double gamma, int sA, int sB, double chiN,
+                       double lamA, double lamB, double lamY, int mirror, int compressible, int mde,
+                       int contour, const double *wA0, const double *wB0) {
+    or_model *m = or_model_create(dim, Lx, Ly, Lz, cs, a, b, c, alpha, beta, gamma, sA, sB, chiN,
+                                  lamA, lamB, 0, mde, contour, wA0, wB0);
+    if (!m) return NULL;
+    m->em = 1;
+    m->mirror = mirror;
+    m->compressible = compressible;
+    m->lamA = lamA; m->lamB = lamB; m->lamY = lamY;
+    m->yita = (double *)calloc(m->M, sizeof(double)); /* Yita(name, cfg, lam): zeros, src/Grid.cc:88-94 */
+    return m;
+}
+
 void or_model_update(or_model *m) {
+    if (m->em) { em_update(m); return; }
     long M = m->M;
     const double *wA = m->wA.data, *wB = m->wB.data;
     /* src/Model_AB.cc:139-145 */
@@ -707,10 +774,15 @@ void or_model_update(or_model *m) {
 }
 
 double or_model_Hw(const or_model *m) {
-    /* src/Model_AB.cc:280-284: uses the fields as they are after update() */
-    for (long i = 0; i < m->M; i++)
+    /* src/Model_AB.cc:265-291: uses the fields as they are after update() */
+    for (long i = 0; i < m->M; i++) {
         m->tmp[i] = m->chiN * m->phiA[i] * m->phiB[i] - m->wA.data[i] * m->phiA[i] -
                     m->wB.data[i] * m->phiB[i];
+        if (m->compressible) {
+            double e = m->phiA[i] + m->phiB[i] - 1.0;
+            m->tmp[i] += m->lamY / 2.0 * e * e;
+        }
+    }
     return or_mean(m->M, m->tmp);
 }
 
@@ -718,7 +790,42 @@ double or_model_Hs(const or_model *m) { return -log(m->Q); } /* :294-296 */
 double or_model_H(const or_model *m) { return or_model_Hw(m) + or_model_Hs(m); }
 double or_model_Q(const or_model *m) { return m->Q; }
 
+static double abs_mean_diff(long M, const double *a, const double *b, double *tmp) {
+    for (long i = 0; i < M; i++) tmp[i] = fabs(a[i] - b[i]);
+    return or_mean((int)M, tmp);
+}
+
 double or_model_residual(const or_model *m) {
+    if (m->em && m->compressible) {
+        /* src/Model_AB.cc:352-367 (abs_quadrature = mean |.| on a regular grid) */
+        long M = m->M;
+        double *g = m->W, *z = m->D, res = 0.0;
+        for (long i = 0; i < M; i++) z[i] = 0.0;
+        for (long i = 0; i < M; i++)
+            g[i] = m->chiN * m->phiB[i] + m->lamY * (m->phiA[i] + m->phiB[i] - 1.0);
+        res += abs_mean_diff(M, g, m->wA.data, m->tmp) /
+               (abs_mean_diff(M, m->wA.data, z, m->tmp) + abs_mean_diff(M, g, z, m->tmp));
+        for (long i = 0; i < M; i++)
+            g[i] = m->chiN * m->phiA[i] + m->lamY * (m->phiA[i] + m->phiB[i] - 1.0);
+        res += abs_mean_diff(M, g, m->wB.data, m->tmp) /
+               (abs_mean_diff(M, m->wB.data, z, m->tmp) + abs_mean_diff(M, g, z, m->tmp));
+        return res / 2.0;
+    }
+    if (m->em) {
+        /* src/Model_AB.cc:326-350, incompressible simple mixing */
+        long M = m->M;
+        double *g = m->W, *z = m->D, res = 0.0;
+        for (long i = 0; i < M; i++) z[i] = 0.0;
+        for (long i = 0; i < M; i++) g[i] = m->chiN * m->phiB[i] + m->yita[i];
+        res += abs_mean_diff(M, g, m->wA.data, m->tmp) /
+               (abs_mean_diff(M, m->wA.data, z, m->tmp) + abs_mean_diff(M, g, z, m->tmp));
+        for (long i = 0; i < M; i++) g[i] = m->chiN * m->phiA[i] + m->yita[i];
+        res += abs_mean_diff(M, g, m->wB.data, m->tmp) /
+               (abs_mean_diff(M, m->wB.data, z, m->tmp) + abs_mean_diff(M, g, z, m->tmp));
+        for (long i = 0; i < M; i++) { g[i] = m->phiA[i] + m->phiB[i]; m->tmp2[i] = 1.0; }
+        res += abs_mean_diff(M, g, m->tmp2, m->tmp) / (1.0 + abs_mean_diff(M, g, z, m->tmp));
+        return res / 3.0;
+    }
     /* src/Model_AB.cc:309-315 */
     double eA1 = fax_d2(&m->wA, m->tmp), eA2 = fax_w2(&m->wA, m->tmp);
     double eB1 = fax_d2(&m->wB, m->tmp), eB2 = fax_w2(&m->wB, m->tmp);
@@ -750,4 +857,71 @@ int or_model_relax(or_model *m, int max_iter, int min_iter,
     if (H_out) *H_out = H;
     if (res_out) *res_out = res;
     return it;
+}
+
+/* ---------------------------------------------------------- string method */
+int or_model_set_fields(or_model *m, const double *wA, const double *wB) {
+    /* Model_AB::resetInString without a size change (src/Model_AB.cc:52-83):
+     * new fields, a fresh (zero) yita */
+    memcpy(m->wA.data, wA, sizeof(double) * m->M);
+    memcpy(m->wB.data, wB, sizeof(double) * m->M);
+    if (m->yita) memset(m->yita, 0, sizeof(double) * m->M);
+    return 0;
+}
+
+void or_string_arclength(int m, long M, const double *wa, double *S) {
+    /* scft/polystring_AB.cc:283-297: S_s = S_{s-1} + sqrt(mean((wa_s - wa_{s-1})^2)), normalised */
+    S[0] = 0.0;
+    for (int s = 1; s < m; s++) {
+        double acc = 0.0;
+        for (long i = 0; i < M; i++) {
+            double d = wa[(long)s * M + i] - wa[(long)(s - 1) * M + i];
+            acc += d * d;
+        }
+        S[s] = S[s - 1] + sqrt(acc / (double)M);
+    }
+    for (int s = 1; s < m; s++) S[s] /= S[m - 1];
+}
+
+void or_spline_weights(int m, const double *S, double *W) {
+    /* Natural cubic spline through (S_j, y_j) evaluated at z/(m-1), as a
+     * linear map y_new = W y (scft/spline.h default: zero second derivative
+     * at both ends).  Second derivatives c solve the tridiagonal system
+     *   h_{i-1} c_{i-1} + 2 (h_{i-1} + h_i) c_i + h_i c_{i+1}
+     *     = 6 ((y_{i+1} - y_i)/h_i - (y_i - y_{i-1})/h_{i-1}),  c_0 = c_{m-1} = 0,
+     * and f(x) = (c_i (x_{i+1}-x)^3 + c_{i+1} (x-x_i)^3)/(6 h_i)
+     *          + (y_i/h_i - c_i h_i/6)(x_{i+1}-x) + (y_{i+1}/h_i - c_{i+1} h_i/6)(x-x_i). */
+    double *h = (double *)malloc(sizeof(double) * m);
+    double *c = (double *)calloc(m, sizeof(double));
+    double *y = (double *)calloc(m, sizeof(double));
+    double *sub = (double *)malloc(sizeof(double) * m), *dia = (double *)malloc(sizeof(double) * m);
+    double *sup = (double *)malloc(sizeof(double) * m), *rhs = (double *)malloc(sizeof(double) * m);
+    for (int i = 0; i + 1 < m; i++) h[i] = S[i + 1] - S[i];
+    for (int j = 0; j < m; j++) {
+        for (int i = 0; i < m; i++) y[i] = (i == j);
+        for (int i = 0; i < m; i++) c[i] = 0.0;
+        int n = m - 2;
+        if (n > 0) {
+            for (int i = 1; i <= n; i++) {
+                sub[i] = h[i - 1]; dia[i] = 2.0 * (h[i - 1] + h[i]); sup[i] = h[i];
+                rhs[i] = 6.0 * ((y[i + 1] - y[i]) / h[i] - (y[i] - y[i - 1]) / h[i - 1]);
+            }
+            for (int i = 2; i <= n; i++) { /* Thomas */
+                double f = sub[i] / dia[i - 1];
+                dia[i] -= f * sup[i - 1];
+                rhs[i] -= f * rhs[i - 1];
+            }
+            c[n] = rhs[n] / dia[n];
+            for (int i = n - 1; i >= 1; i--) c[i] = (rhs[i] - sup[i] * c[i + 1]) / dia[i];
+        }
+        for (int z = 0; z < m; z++) {
+            double x = (m > 1) ? (double)z / (m - 1) : 0.0;
+            int i = 0;
+            while (i < m - 2 && x > S[i + 1]) i++;
+            double a = S[i + 1] - x, b = x - S[i], hi = h[i];
+            W[(long)z * m + j] = (c[i] * a * a * a + c[i + 1] * b * b * b) / (6.0 * hi) +
+                                 (y[i] / hi - c[i] * hi / 6.0) * a + (y[i + 1] / hi - c[i + 1] * hi / 6.0) * b;
+        }
+    }
+    free(h); free(c); free(y); free(sub); free(dia); free(sup); free(rhs);
 }

@@ -124,6 +124,24 @@ int or_model_relax(or_model *m, int max_iter, int min_iter,
                    int record_interval, double thresh,
                    double *H_out, double *res_out);
 
+/* ---- simple mixing (EM) and the string method ------------------------ */
+/* Model_AB with simple mixing (Field/Yita updates, src/Model_AB.cc:254-258,
+ * residual :326-350), as the string driver runs it; mirror = 1 applies the
+ * NBC-by-PBC mirroring of src/Model_AB.cc:148-182 before each update.
+ * lamY is lam[2]: the yita mixing parameter, or with compressible = 1 the
+ * penalty lamYita of eta = lamYita (phiA + phiB - 1) (src/Model_AB.cc:205-229). */
+or_model *or_em_create(int dim, int Lx, int Ly, int Lz, int cs, double a, double b, double c,
+                       double alpha, double beta, double gamma, int sA, int sB, double chiN,
+                       double lamA, double lamB, double lamY, int mirror, int compressible, int mde,
+                       int contour, const double *wA0, const double *wB0);
+/* New fields and a zero yita (Model_AB::resetInString, src/Model_AB.cc:52-83). */
+int or_model_set_fields(or_model *m, const double *wA, const double *wB);
+/* Normalised arc length of a string of m images (scft/polystring_AB.cc:283-297). */
+void or_string_arclength(int m, long M, const double *wa, double *S);
+/* W (m x m, row-major): natural cubic spline through (S_j, y_j) evaluated at
+ * z/(m-1), y_new = W y (scft/polystring_AB.cc:299-329, scft/spline.h). */
+void or_spline_weights(int m, const double *S, double *W);
+
 /* Number of OpenMP threads the oracle uses (1 if built without OpenMP). */
 int or_num_threads(void);
 void or_set_num_threads(int n);

@@ -33,7 +33,8 @@ EXPORTS = [
     "po_model_create", "po_model_destroy", "po_model_update", "po_model_observe", "po_model_get",
     "po_model_relax", "po_model_contour_steps", "po_model_stress", "po_model_set_cell",
     "po_model_relax_cell", "po_model_set_adjoint", "po_model_last_stress", "po_model_grad_w",
-    "po_model_set_fields",
+    "po_model_set_fields", "po_string_create", "po_string_destroy", "po_string_iterate", "po_string_get",
+    "po_string_contour_steps",
     "po_model_set_profiling", "po_model_profile",
 ]
 
@@ -49,6 +50,13 @@ class ModelDesc(ctypes.Structure):
     _fields_ = [("grid", GridDesc), ("sA", ctypes.c_int), ("sB", ctypes.c_int), ("chiN", ctypes.c_double),
                 ("lamA", ctypes.c_double), ("lamB", ctypes.c_double), ("n_anderson", ctypes.c_int),
                 ("algo", ctypes.c_int), ("contour", ctypes.c_int)]
+
+
+class StringDesc(ctypes.Structure):
+    _fields_ = [("grid", GridDesc), ("sA", ctypes.c_int), ("sB", ctypes.c_int), ("chiN", ctypes.c_double),
+                ("lamA", ctypes.c_double), ("lamB", ctypes.c_double), ("lamY", ctypes.c_double),
+                ("algo", ctypes.c_int), ("contour", ctypes.c_int), ("compressible", ctypes.c_int),
+                ("mirror", ctypes.c_int), ("m", ctypes.c_int)]
 
 
 class POError(RuntimeError):
@@ -89,6 +97,9 @@ def load():
         "po_model_relax_cell": ([_vp, i, d, i, i, i, d, _dp, _dp, _dp, ctypes.POINTER(i)], i),
         "po_model_set_adjoint": ([_vp, i], i), "po_model_last_stress": ([_vp, _dp], i),
         "po_model_grad_w": ([_vp, _dp, _dp, _dp], i), "po_model_set_fields": ([_vp, _dp, _dp], i),
+        "po_string_create": ([_vp, ctypes.POINTER(StringDesc), _dp, _dp, ctypes.POINTER(_vp)], i),
+        "po_string_destroy": ([_vp], i), "po_string_iterate": ([_vp, i, i, i, d, _dp, _dp], i),
+        "po_string_get": ([_vp, i, _dp], i), "po_string_contour_steps": ([_vp], ctypes.c_long),
         "po_model_set_profiling": ([_vp, i], i), "po_model_profile": ([_vp, _dp], i),
     }
     for name, (args, res) in sig.items():
@@ -297,3 +308,46 @@ class ModelAB:
         o = np.zeros(4)
         _check(load().po_model_profile(self.h, _p(o)))
         return dict(prop_ms=o[0], rest_ms=o[1], iters=int(o[2]), lockstep_steps=int(o[3]))
+
+
+class String:
+    """The string method (scft/polystring_AB.cc) with all m images on the GPU.
+
+    wA, wB: (m, Lx, Ly, Lz) initial images; images 0 and m-1 are the pinned
+    ends.  lam = (lamA, lamB, lamYita)."""
+
+    def __init__(self, ctx, wA, wB, *, cell, Ms, chiN, lam=(0.1, 0.1, 1.0), algo="RQM4", contour="simpson",
+                 compressible=False, mirror=False, dim=None):
+        wA = np.ascontiguousarray(wA, dtype=np.float64)
+        wB = np.ascontiguousarray(wB, dtype=np.float64)
+        self.m = wA.shape[0]
+        self.shape = wA.shape[1:]
+        self.ctx = ctx
+        d = StringDesc(grid_desc(self.shape, cell, dim), int(Ms[0]), int(Ms[1]), float(chiN), float(lam[0]),
+                       float(lam[1]), float(lam[2]), PO_RQM4 if algo.upper() == "RQM4" else PO_OS2,
+                       PO_QUAD_TRAP if contour.lower() == "trap" else PO_QUAD_AUTO, int(bool(compressible)),
+                       int(bool(mirror)), self.m)
+        h = _vp()
+        _check(load().po_string_create(ctx.h, ctypes.byref(d), _p(wA), _p(wB), ctypes.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            load().po_string_destroy(self.h)
+            self.h = None
+
+    def iterate(self, max_iter=3, min_iter=1, record_interval=1, thresh=1e-10):
+        """One string iteration; returns (H per image, normalised arc length S)."""
+        H = np.zeros(self.m)
+        S = np.zeros(self.m)
+        _check(load().po_string_iterate(self.h, int(max_iter), int(min_iter), int(record_interval), float(thresh),
+                                        _p(H), _p(S)))
+        return H, S
+
+    def get(self, which):
+        out = np.zeros((self.m,) + tuple(self.shape))
+        _check(load().po_string_get(self.h, FIELDS[which], _p(out)))
+        return out
+
+    def contour_steps(self):
+        return load().po_string_contour_steps(self.h)

@@ -13,509 +13,16 @@
 #include <string>
 #include <vector>
 
-#include "../../include/polystring_amd.h"
-#include "anderson.hpp"
-#include "engine.hpp"
+#include "runtime.hpp"
 
 namespace po {
 
 static thread_local std::string g_err;
 
-static int fail(int code, const std::string &msg) {
+int fail(int code, const std::string &msg) {
     g_err = msg;
     return code;
 }
-
-#define PO_HIP(x)                                                                                   \
-    do {                                                                                            \
-        hipError_t e_ = (x);                                                                        \
-        if (e_ != hipSuccess) return fail(PO_ERR_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
-    } while (0)
-
-template <typename T> struct DBuf {
-    T *p = nullptr;
-    size_t n = 0;
-    DBuf() = default;
-    DBuf(const DBuf &) = delete;
-    DBuf &operator=(const DBuf &) = delete;
-    ~DBuf() { release(); }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-    hipError_t alloc(size_t count) {
-        release();
-        n = count;
-        return hipMalloc((void **)&p, sizeof(T) * (count ? count : 1));
-    }
-};
-
-// ----------------------------------------------------------------- cell --
-// Shape matrices of UnitCell::calc_shape_matrix{1,2,3} (src/UnitCell.cc:142-296):
-// lengths l = (a, b, c) and reciprocal matrix g (row-major, 2 pi units).
-struct Cell {
-    int dim;
-    double l[3];
-    double g[9];
-    bool ortho;
-};
-
-static void cross3(const double *u, const double *v, double *o) {
-    o[0] = u[1] * v[2] - u[2] * v[1];
-    o[1] = u[2] * v[0] - u[0] * v[2];
-    o[2] = u[0] * v[1] - u[1] * v[0];
-}
-
-static int make_cell(const po_grid_desc &gd, Cell &c) {
-    const double pi = 3.14159265358979323846264338327950288;
-    const double pi2 = pi / 2.0, pi3 = 2.0 * pi / 3.0, twopi = 2.0 * pi;
-    double a = gd.a, b = gd.b, cc = gd.c, al = gd.alpha, be = gd.beta, ga = gd.gamma;
-    std::memset(&c, 0, sizeof(c));
-    c.dim = gd.dim;
-    if (gd.dim == 1) {
-        if (!(a > 0)) return fail(PO_ERR_INVALID, "cell length a must be positive");
-        c.l[0] = a;
-        c.g[0] = twopi / a;
-        c.g[4] = twopi;
-        c.g[8] = twopi;
-        c.ortho = true;
-        return PO_OK;
-    }
-    if (gd.dim == 2) {
-        if (!(a > 0)) return fail(PO_ERR_INVALID, "cell length a must be positive");
-        switch (gd.crystal) {
-        case 1: b = a; ga = pi2; break;                                       // SQUARE
-        case 2: if (!(b > 0)) return fail(PO_ERR_INVALID, "bad RECTANGULAR cell"); ga = pi2; break;
-        case 3: b = a; ga = pi3; break;                                       // HEXAGONAL
-        default: if (!(b > 0) || !(ga > 0)) return fail(PO_ERR_INVALID, "bad 2-D cell"); break;
-        }
-        const double A[3] = {a, 0, 0}, B[3] = {b * std::cos(ga), b * std::sin(ga), 0};
-        double acb[3], as[3], bs[3];
-        cross3(A, B, acb);
-        const double denom = acb[0] * acb[0] + acb[1] * acb[1] + acb[2] * acb[2];
-        cross3(B, acb, as);
-        cross3(acb, A, bs);
-        c.l[0] = a;
-        c.l[1] = b;
-        c.g[0] = twopi * as[0] / denom; c.g[1] = twopi * bs[0] / denom;
-        c.g[3] = twopi * as[1] / denom; c.g[4] = twopi * bs[1] / denom;
-        c.g[8] = twopi;
-        c.ortho = (ga == pi2);
-        return PO_OK;
-    }
-    if (gd.dim != 3) return fail(PO_ERR_INVALID, "dim must be 1, 2 or 3");
-    switch (gd.crystal) {
-    case 5: b = a; cc = a; al = be = ga = pi2; break;      // CUBIC
-    case 6: b = a; al = be = ga = pi2; break;              // TETRAGONAL
-    case 7: al = be = ga = pi2; break;                     // ORTHORHOMBIC
-    case 3: b = a; al = be = pi2; ga = pi3; break;         // HEXAGONAL
-    case 8: b = a; cc = a; al = ga; be = ga; break;        // TRIGONAL
-    case 9: al = pi2; ga = pi2; break;                     // MONOCLINIC
-    default: return fail(PO_ERR_INVALID, "unknown 3-D crystal system");
-    }
-    const double A[3] = {a, 0, 0}, B[3] = {b * std::cos(ga), b * std::sin(ga), 0};
-    const double cx = cc * std::cos(be), cy = cc * (std::cos(al) - std::cos(be) * std::cos(ga));
-    const double C[3] = {cx, cy, std::sqrt(cc * cc - cx * cx - cy * cy)};
-    double bcc[3], cca[3], acb[3];
-    cross3(B, C, bcc);
-    cross3(C, A, cca);
-    cross3(A, B, acb);
-    const double denom = A[0] * bcc[0] + A[1] * bcc[1] + A[2] * bcc[2];
-    for (int i = 0; i < 3; ++i) {
-        c.g[3 * i + 0] = twopi * bcc[i] / denom;
-        c.g[3 * i + 1] = twopi * cca[i] / denom;
-        c.g[3 * i + 2] = twopi * acb[i] / denom;
-    }
-    c.l[0] = a;
-    c.l[1] = b;
-    c.l[2] = cc;
-    c.ortho = (al == pi2 && be == pi2 && ga == pi2);
-    return PO_OK;
-}
-
-// minimum-image index r = i for i < L/2+1, else L - i (src/UnitCell.cc:399-412)
-static inline int rmin(int i, int L) { return i < L / 2 + 1 ? i : L - i; }
-
-// ----------------------------------------------------------- k factors --
-// K(k) = exp(h * (-|k|^2)).  Orthogonal tables (RQM4 always, and OS2 on an
-// orthogonal cell) are separable: K = fx[kx] fy[ky] fz[kz] with
-// fx[i] = exp(h * (-4 pi^2 / lx^2) * r_i^2) (src/UnitCell.cc:368-418).  OS2 on
-// a general cell uses the full half-spectrum table of
-// exp(-h * min over 27 images |g (n + shift)|^2) (src/UnitCell.cc:321-359).
-template <typename T> struct KTable {
-    DBuf<T> fx, fy, fz, tab;
-    // dK/dl_d = K g_d with g_d[r] = 2 h (2 pi r / l_d)^2 / l_d (separable tables only)
-    DBuf<T> gx, gy, gz;
-    XDesc<T> desc(cx<T> *S, cx<T> *Sout = nullptr) const { return XDesc<T>{S, fx.p, fy.p, fz.p, tab.p, Sout}; }
-};
-
-// (Re)allocate only on a size change, so descriptors that hold the pointer
-// stay valid when a table is refilled for a new cell.
-template <typename T> static hipError_t ensure(DBuf<T> &b, size_t n) { return b.n == n && b.p ? hipSuccess : b.alloc(n); }
-
-template <typename T> static int upload(DBuf<T> &b, const std::vector<T> &h, hipStream_t st) {
-    PO_HIP(ensure(b, h.size()));
-    PO_HIP(hipMemcpyAsync(b.p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, st));
-    return PO_OK;
-}
-
-template <typename T>
-static int build_ktable(KTable<T> &kt, const Shape &sh, const Cell &cell, bool general, double h, hipStream_t st) {
-    const double pi = 3.14159265358979323846264338327950288;
-    if (!general) {
-        double cc[3] = {0, 0, 0}, gc[3] = {0, 0, 0};
-        for (int d = 0; d < cell.dim; ++d) {
-            cc[d] = -4.0 * pi * pi / (cell.l[d] * cell.l[d]);
-            gc[d] = -2.0 * h * cc[d] / cell.l[d];
-        }
-        const int N[3] = {sh.Lx, sh.Ly, sh.Hz}, P[3] = {sh.Lx, sh.Ly, sh.Lz};
-        DBuf<T> *F[3] = {&kt.fx, &kt.fy, &kt.fz}, *G[3] = {&kt.gx, &kt.gy, &kt.gz};
-        int rc;
-        for (int d = 0; d < 3; ++d) {
-            std::vector<T> f(N[d]), g(N[d]);
-            for (int i = 0; i < N[d]; ++i) {
-                const double r = rmin(i, P[d]);
-                f[i] = (T)std::exp(h * (cc[d] * r * r));
-                g[i] = (T)(gc[d] * r * r);
-            }
-            if ((rc = upload(*F[d], f, st)) || (rc = upload(*G[d], g, st))) return rc;
-        }
-        PO_HIP(hipStreamSynchronize(st));
-        return PO_OK;
-    }
-    const int N[3] = {sh.Lx, sh.Ly, sh.Lz};
-    std::vector<T> tab((size_t)sh.Mh);
-    for (int i = 0; i < sh.Lx; ++i)
-        for (int j = 0; j < sh.Ly; ++j)
-            for (int k = 0; k < sh.Hz; ++k) {
-                double q2min = 1e99;
-                for (int a = 1; a > -2; --a)
-                    for (int b = 1; b > -2; --b)
-                        for (int c = 1; c > -2; --c) {
-                            const double bq[3] = {(double)(a * N[0]) + i, (double)(b * N[1]) + j,
-                                                  (double)(c * N[2]) + k};
-                            double q2 = 0.0;
-                            for (int r = 0; r < 3; ++r) {
-                                const double qt = cell.g[3 * r] * bq[0] + cell.g[3 * r + 1] * bq[1] +
-                                                  cell.g[3 * r + 2] * bq[2];
-                                q2 += qt * qt;
-                            }
-                            if (q2 < q2min) q2min = q2;
-                        }
-                tab[((size_t)i * sh.Ly + j) * sh.Hz + k] = (T)std::exp(-h * q2min);
-            }
-    int rc;
-    if ((rc = upload(kt.tab, tab, st))) return rc;
-    PO_HIP(hipStreamSynchronize(st));
-    return PO_OK;
-}
-
-}  // namespace po
-
-// ----------------------------------------------------------------- context --
-struct po_ctx {
-    int device = 0;
-    int precision = PO_FP64;
-    hipStream_t stream = nullptr;
-    std::map<int, std::unique_ptr<po::DBuf<double2>>> roots64;
-    std::map<int, std::unique_ptr<po::DBuf<float2>>> roots32;
-};
-
-namespace po {
-
-template <typename T> struct RootsMap;
-template <> struct RootsMap<double> {
-    static std::map<int, std::unique_ptr<DBuf<double2>>> &get(po_ctx *c) { return c->roots64; }
-};
-template <> struct RootsMap<float> {
-    static std::map<int, std::unique_ptr<DBuf<float2>>> &get(po_ctx *c) { return c->roots32; }
-};
-
-// N-th roots of unity exp(-2 pi i m / N), computed in long double, cached per context
-template <typename T> static int roots_for(po_ctx *ctx, int N, const cx<T> **out) {
-    auto &m = RootsMap<T>::get(ctx);
-    const int key = N < 1 ? 1 : N;
-    auto it = m.find(key);
-    if (it == m.end()) {
-        std::vector<cx<T>> h(key);
-        for (int i = 0; i < key; ++i) {
-            const long double ang = -2.0L * 3.141592653589793238462643383279502884L * i / key;
-            h[i].x = (T)cosl(ang);
-            h[i].y = (T)sinl(ang);
-        }
-        auto buf = std::make_unique<DBuf<cx<T>>>();
-        PO_HIP(buf->alloc(key));
-        PO_HIP(hipMemcpy(buf->p, h.data(), sizeof(cx<T>) * key, hipMemcpyHostToDevice));
-        it = m.emplace(key, std::move(buf)).first;
-    }
-    *out = it->second->p;
-    return PO_OK;
-}
-
-template <typename T> static int make_roots(po_ctx *ctx, const Shape &sh, Roots<T> &rt) {
-    int rc;
-    if ((rc = roots_for<T>(ctx, sh.Lx, &rt.x))) return rc;
-    if ((rc = roots_for<T>(ctx, sh.Ly, &rt.y))) return rc;
-    if ((rc = roots_for<T>(ctx, sh.Lz / 2, &rt.zh))) return rc;
-    if ((rc = roots_for<T>(ctx, sh.Lz, &rt.z))) return rc;
-    return PO_OK;
-}
-
-static int make_shape(const po_grid_desc &g, Shape &sh) {
-    if (g.Lx < 1 || g.Ly < 1 || g.Lz < 1) return fail(PO_ERR_INVALID, "grid sizes must be positive");
-    if (!plane_supported(g.Ly, g.Lz) || !xpass_supported(g.Lx))
-        return fail(PO_ERR_INVALID, "grid " + std::to_string(g.Lx) + "x" + std::to_string(g.Ly) + "x" +
-                                        std::to_string(g.Lz) + " has no compiled kernels");
-    sh.Lx = g.Lx;
-    sh.Ly = g.Ly;
-    sh.Lz = g.Lz;
-    sh.Hz = g.Lz == 1 ? 1 : g.Lz / 2 + 1;
-    sh.M = (long)g.Lx * g.Ly * g.Lz;
-    sh.Mh = (long)g.Lx * g.Ly * sh.Hz;
-    return PO_OK;
-}
-
-// ------------------------------------------------------- chain programs --
-// A lane is a sequence of propagator segments solved back to back, the head
-// of each segment being the tail of the previous one (qB after qA, qAc after
-// qBc: src/Model_AB.cc:139-145).  Lanes of one program advance in lockstep,
-// one launch per pass for all lanes.
-template <typename T> struct Segment {
-    T *q;
-    int len;
-    const T *e2, *e4;            // exp(-w ds/2), exp(-w ds/4)
-    const KTable<T> *kfull, *khalf;  // K for h = ds and h = ds/2
-};
-
-template <typename T> struct Lane {
-    std::vector<Segment<T>> segs;
-    cx<T> *Sf, *Sh;
-};
-
-// Fused adjoint stress of a two-lane program (lane 0 the forward chain,
-// lane 1 the complementary chain, both N steps with N even).  Step s of the
-// forward chain pairs with step N-1-s of the complementary chain.  In the
-// first half the x-pencil passes store each lane's three full spectra per step
-// (F E2 q, F E4 q, F E4 S_h q); in the second half each pass dots its live
-// spectrum with the partner's stored one (kinds 0-0, 1-2, 2-1), so every
-// step pair is visited exactly once and no spectrum is recomputed.
-template <typename T> struct AdjointArena {
-    cx<T> *spec = nullptr;        // (N/2) x 2 lanes x kinds (3 RQM4, 1 OS2) x elems
-    long elems = 0;               // complex per stored spectrum
-    double *partials = nullptr;   // per dot descriptor: nblk x 3
-    int nblk = 0;
-};
-
-template <typename T> struct Program {
-    // FP64 takes exp(-w ds/2) as the square of exp(-w ds/4) (two fewer
-    // slice reads per step, a rounding-level change); FP32 keeps both tables,
-    // whose independent roundings hold the mixed-precision error of H lower.
-    static constexpr int kE2Sq = sizeof(T) == 8 ? 1 : 0;
-    struct Launch {
-        bool plane;
-        int off, n;
-    };
-    DBuf<PlaneDesc<T>> pd;
-    DBuf<XDesc<T>> xd;
-    std::vector<Launch> seq;
-    int ndot = 0;  // dot descriptors of a fused adjoint program
-
-    int build(int algo, const std::vector<Lane<T>> &lanes, hipStream_t st, const AdjointArena<T> *adj = nullptr) {
-        const int L = (int)lanes.size();
-        int nsteps = -1;
-        for (const auto &ln : lanes) {
-            int s = 0;
-            for (const auto &sg : ln.segs) {
-                if (sg.len < 2) return fail(PO_ERR_INVALID, "propagator length must be >= 2");
-                s += sg.len - 1;
-            }
-            if (nsteps >= 0 && s != nsteps) return fail(PO_ERR_INVALID, "lockstep lanes need equal step counts");
-            nsteps = s;
-        }
-        std::vector<PlaneDesc<T>> P;
-        std::vector<XDesc<T>> X;
-        seq.clear();
-        auto at = [&](int lane, int t, int &si, int &s) {
-            si = 0;
-            s = t;
-            while (s >= lanes[lane].segs[si].len - 1) { s -= lanes[lane].segs[si].len - 1; ++si; }
-        };
-        auto push_p = [&](std::vector<PlaneDesc<T>> &v) { seq.push_back({true, (int)P.size(), (int)v.size()}); P.insert(P.end(), v.begin(), v.end()); };
-        auto push_x = [&](std::vector<XDesc<T>> &v) { seq.push_back({false, (int)X.size(), (int)v.size()}); X.insert(X.end(), v.begin(), v.end()); };
-        const bool rqm4 = algo == PO_RQM4;
-        if (adj && (L != 2 || nsteps % 2 != 0))
-            return fail(PO_ERR_INVALID, "the fused adjoint needs two lanes with an even step count");
-        ndot = 0;
-        // attach the adjoint store / dot of (step t, lane l, kind k) to an x-pass descriptor
-        auto adjoin = [&](XDesc<T> x, const KTable<T> &kt, int t, int l, int k) {
-            if (!adj) return x;
-            const int half = nsteps / 2, nk = rqm4 ? 3 : 1;
-            x.gx = kt.gx.p;
-            x.gy = kt.gy.p;
-            x.gz = kt.gz.p;
-            if (t < half) {
-                x.Sstore = adj->spec + ((size_t)(t * 2 + l) * nk + k) * adj->elems;
-            } else {
-                static const int partner[3] = {0, 2, 1};
-                const int tp = nsteps - 1 - t;
-                x.Sdot = adj->spec + ((size_t)(tp * 2 + (1 - l)) * nk + partner[k]) * adj->elems;
-                x.coef = rqm4 ? (k == 0 ? -1.0 / 3.0 : 4.0 / 3.0) : 1.0;
-                x.partials = adj->partials + (size_t)ndot * adj->nblk * 3;
-                ++ndot;
-            }
-            return x;
-        };
-        {
-            std::vector<PlaneDesc<T>> v;
-            for (int l = 0; l < L; ++l) {
-                const auto &sg = lanes[l].segs[0];
-                PlaneDesc<T> d{};
-                d.qin = sg.q;
-                d.out0 = lanes[l].Sf;
-                d.pre0 = sg.e2;
-                if (rqm4) { d.out1 = lanes[l].Sh; d.pre1 = sg.e4; d.e2sq = kE2Sq; }
-                v.push_back(d);
-            }
-            push_p(v);
-        }
-        for (int t = 0; t < nsteps; ++t) {
-            std::vector<XDesc<T>> x2, x1;
-            std::vector<PlaneDesc<T>> mid, fin;
-            for (int l = 0; l < L; ++l) {
-                int si, s;
-                at(l, t, si, s);
-                const auto &sg = lanes[l].segs[si];
-                PlaneDesc<T> f{};
-                f.in0 = lanes[l].Sf;
-                f.post0 = sg.e2;
-                if (rqm4) {
-                    x2.push_back(adjoin(sg.kfull->desc(lanes[l].Sf), *sg.kfull, t, l, 0));
-                    x2.push_back(adjoin(sg.khalf->desc(lanes[l].Sh), *sg.khalf, t, l, 1));
-                    PlaneDesc<T> m{};
-                    m.in0 = lanes[l].Sh;
-                    m.post0 = sg.e4;
-                    m.out0 = lanes[l].Sh;
-                    m.pre0 = sg.e4;
-                    mid.push_back(m);
-                    x1.push_back(adjoin(sg.khalf->desc(lanes[l].Sh), *sg.khalf, t, l, 2));
-                    f.in1 = lanes[l].Sh;
-                    f.post1 = sg.e4;
-                    f.e2sq = kE2Sq;
-                } else {
-                    x2.push_back(adjoin(sg.kfull->desc(lanes[l].Sf), *sg.kfull, t, l, 0));
-                }
-                f.qout = sg.q + (size_t)(s + 1) * slice_elems;
-                if (s + 1 == sg.len - 1 && si + 1 < (int)lanes[l].segs.size()) f.qout2 = lanes[l].segs[si + 1].q;
-                if (t + 1 < nsteps) {
-                    int sn, ssn;
-                    at(l, t + 1, sn, ssn);
-                    const auto &nx = lanes[l].segs[sn];
-                    f.out0 = lanes[l].Sf;
-                    f.pre0 = nx.e2;
-                    if (rqm4) { f.out1 = lanes[l].Sh; f.pre1 = nx.e4; }
-                }
-                fin.push_back(f);
-            }
-            push_x(x2);
-            if (rqm4) {
-                push_p(mid);
-                push_x(x1);
-            }
-            push_p(fin);
-        }
-        PO_HIP(pd.alloc(P.size()));
-        PO_HIP(xd.alloc(X.size() ? X.size() : 1));
-        PO_HIP(hipMemcpyAsync(pd.p, P.data(), sizeof(PlaneDesc<T>) * P.size(), hipMemcpyHostToDevice, st));
-        if (!X.empty()) PO_HIP(hipMemcpyAsync(xd.p, X.data(), sizeof(XDesc<T>) * X.size(), hipMemcpyHostToDevice, st));
-        PO_HIP(hipStreamSynchronize(st));
-        return PO_OK;
-    }
-
-    int run(const Shape &sh, const Roots<T> &rt, hipStream_t st) const {
-        for (const auto &l : seq) {
-            hipError_t e = l.plane ? launch_plane<T>(sh, rt, pd.p + l.off, l.n, st, plane_variant)
-                                   : launch_xpass<T>(sh, rt, xd.p + l.off, l.n, st);
-            if (e != hipSuccess) return fail(PO_ERR_DEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
-        }
-        return PO_OK;
-    }
-
-    // Autotuner: times the whole program once per plane-kernel variant (the
-    // workgroup size of the yz-plane kernel) on the live buffers and keeps
-    // the fastest.  Every slice and spectrum the program writes is rewritten
-    // by the next real run, so tuning leaves no trace in the results.
-    int tune(const Shape &sh, const Roots<T> &rt, hipStream_t st) {
-        const int nvar = (long)sh.Ly * sh.Lz >= 8192 ? 2 : 1;
-        if (nvar == 1) return PO_OK;
-        // PO_PLANE_VARIANT=0|1 pins the choice (reproducible launch shapes)
-        if (const char *env = std::getenv("PO_PLANE_VARIANT")) {
-            const int v = std::atoi(env);
-            if (v < 0 || v >= nvar) return fail(PO_ERR_INVALID, "PO_PLANE_VARIANT out of range");
-            plane_variant = v;
-            return PO_OK;
-        }
-        hipEvent_t a, b;
-        PO_HIP(hipEventCreate(&a));
-        PO_HIP(hipEventCreate(&b));
-        float best = 1e30f;
-        int pick = 0;
-        for (int v = 0; v < nvar; ++v) {
-            plane_variant = v;
-            int rc = run(sh, rt, st);  // warm-up
-            if (rc) return rc;
-            PO_HIP(hipEventRecord(a, st));
-            for (int rep = 0; rep < 2 && rc == PO_OK; ++rep) rc = run(sh, rt, st);
-            if (rc) return rc;
-            PO_HIP(hipEventRecord(b, st));
-            PO_HIP(hipEventSynchronize(b));
-            float ms = 0.f;
-            PO_HIP(hipEventElapsedTime(&ms, a, b));
-            tuned_ms[v] = ms / 2;
-            if (ms < best) { best = ms; pick = v; }
-        }
-        (void)hipEventDestroy(a);
-        (void)hipEventDestroy(b);
-        plane_variant = pick;
-        return PO_OK;
-    }
-
-    // With PO_GRAPH=1 the launch sequence is captured once into a hipGraph
-    // and replayed; every kernel argument (descriptor table offsets) is
-    // fixed, so one capture serves every later call.  Direct launches are the
-    // default: replay measured no faster at 128^3 and direct launches are
-    // what the profiler traces reliably.
-    int run_graph(const Shape &sh, const Roots<T> &rt, hipStream_t st) {
-        if (use_graph < 0) {
-            const char *env = std::getenv("PO_GRAPH");
-            use_graph = (env && env[0] == '1') ? 1 : 0;
-        }
-        if (!use_graph) return run(sh, rt, st);
-        if (!exec) {
-            hipGraph_t g = nullptr;
-            PO_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-            int rc = run(sh, rt, st);
-            hipError_t e = hipStreamEndCapture(st, &g);
-            if (rc) return rc;
-            if (e != hipSuccess) return fail(PO_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e));
-            e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            if (e != hipSuccess) return fail(PO_ERR_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(e));
-        }
-        PO_HIP(hipGraphLaunch(exec, st));
-        return PO_OK;
-    }
-
-    ~Program() {
-        if (exec) (void)hipGraphExecDestroy(exec);
-    }
-
-    long slice_elems = 0;  // M, set before build()
-    int plane_variant = 0;
-    float tuned_ms[2] = {0.f, 0.f};
-    int use_graph = -1;
-    hipGraphExec_t exec = nullptr;
-};
 
 }  // namespace po
 

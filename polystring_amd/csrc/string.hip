@@ -1,0 +1,533 @@
+// string.hip -- the string method on one GPU: all images of the string are
+// advanced together, two propagator chains per image as lanes of one launch
+// program, with batched simple-mixing (EM) updates, arc-length
+// parametrisation and a cubic-spline redistribution of the images
+// (scft/polystring_AB.cc:111-329; Model_AB EM branch src/Model_AB.cc:136-261).
+#include "runtime.hpp"
+
+namespace po {
+
+constexpr int kStrMaxImages = 64;
+constexpr int kStrThreads = 256;
+constexpr int kStrBlocks = 256;  // reduction workgroups per image
+enum { EM_HW = 0, EM_RA, EM_XA, EM_BA, EM_RB, EM_XB, EM_BB, EM_RY, EM_BY, EM_NV };
+
+__device__ __forceinline__ double str_block_sum(double v, double *sh) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += sh[i];
+    return s;  // valid in thread 0
+}
+
+// part[img][blk] = partial sums of the tail slice of image img (Q = mean).
+template <typename T>
+__global__ void __launch_bounds__(kStrThreads) k_str_tail(const T *__restrict__ q, long img_stride, long M,
+                                                          double *part) {
+    __shared__ double sh[kStrThreads / 64];
+    const T *x = q + blockIdx.y * img_stride;
+    double s = 0.0;
+    for (long i = blockIdx.x * (long)kStrThreads + threadIdx.x; i < M; i += (long)gridDim.x * kStrThreads)
+        s += (double)x[i];
+    s = str_block_sum(s, sh);
+    if (threadIdx.x == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = s;
+}
+
+// out[r] = sum_b part[r][b] / div, one thread per row, blocks in order
+__global__ void k_str_rows(const double *__restrict__ part, int nrow, int nblk, double div, double *out, int ostride) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrow) return;
+    double s = 0.0;
+    for (int b = 0; b < nblk; ++b) s += part[(long)r * nblk + b];
+    out[(long)r * ostride] = s / div;
+}
+
+struct EmArgs {
+    double *wA, *wB, *yita;
+    const double *phiA, *phiB;
+    const int *active;
+    long M;
+    double chiN, lamA, lamB, lamY;
+    int compressible;
+    double cA2, cA4, cB2, cB4;  // -ds/2, -ds/4 per block
+    double *part;               // [img][blk][EM_NV]
+};
+
+// Simple mixing of every active image (src/Model_AB.cc:205-229,254-258;
+// Field::update src/Field.cc:51-54, Yita::update src/Yita.cc:49-51), the next
+// Boltzmann factors, and per-workgroup partial sums of Hw (new fields, old
+// densities, src/Model_AB.cc:265-291) and of the EM residual terms
+// (src/Model_AB.cc:326-367).
+template <typename T>
+__global__ void __launch_bounds__(kStrThreads) k_str_em(EmArgs a, T *eA2, T *eA4, T *eB2, T *eB4) {
+    __shared__ double sh[kStrThreads / 64];
+    const int img = blockIdx.y;
+    if (!a.active[img]) return;
+    const long o = (long)img * a.M;
+    double acc[EM_NV];
+#pragma unroll
+    for (int v = 0; v < EM_NV; ++v) acc[v] = 0.0;
+    for (long i = blockIdx.x * (long)kStrThreads + threadIdx.x; i < a.M; i += (long)gridDim.x * kStrThreads) {
+        const long j = o + i;
+        const double pa = a.phiA[j], pb = a.phiB[j], inc = pa + pb - 1.0;
+        double y;
+        if (a.compressible) {
+            y = a.lamY * inc;
+        } else {
+            y = a.yita[j] + a.lamY * inc;
+            a.yita[j] = y;
+        }
+        const double ta = a.chiN * pb + y, tb = a.chiN * pa + y;
+        const double wa = a.wA[j] * (1.0 - a.lamA) + a.lamA * ta;
+        const double wb = a.wB[j] * (1.0 - a.lamB) + a.lamB * tb;
+        a.wA[j] = wa;
+        a.wB[j] = wb;
+        eA2[j] = (T)exp(a.cA2 * wa);
+        eA4[j] = (T)exp(a.cA4 * wa);
+        eB2[j] = (T)exp(a.cB2 * wb);
+        eB4[j] = (T)exp(a.cB4 * wb);
+        double hw = a.chiN * pa * pb - wa * pa - wb * pb;
+        if (a.compressible) hw += a.lamY / 2.0 * inc * inc;
+        acc[EM_HW] += hw;
+        // residual terms: g = target field after the update (yita new)
+        acc[EM_RA] += fabs(ta - wa);
+        acc[EM_XA] += fabs(wa);
+        acc[EM_BA] += fabs(ta);
+        acc[EM_RB] += fabs(tb - wb);
+        acc[EM_XB] += fabs(wb);
+        acc[EM_BB] += fabs(tb);
+        acc[EM_RY] += fabs(inc);
+        acc[EM_BY] += fabs(pa + pb);
+    }
+    double *out = a.part + ((long)img * gridDim.x + blockIdx.x) * EM_NV;
+#pragma unroll
+    for (int v = 0; v < EM_NV; ++v) {
+        const double s = str_block_sum(acc[v], sh);
+        if (threadIdx.x == 0) out[v] = s;
+    }
+}
+
+// obs[img] = {H, residual, Q} of every active image from the partials.
+__global__ void k_str_obs(const double *__restrict__ part, int nblk, long M, const int *active, int compressible,
+                          double *obs, int m) {
+    const int img = blockIdx.x * blockDim.x + threadIdx.x;
+    if (img >= m || !active[img]) return;
+    double s[EM_NV];
+    for (int v = 0; v < EM_NV; ++v) s[v] = 0.0;
+    for (int b = 0; b < nblk; ++b)
+        for (int v = 0; v < EM_NV; ++v) s[v] += part[((long)img * nblk + b) * EM_NV + v];
+    const double inv = 1.0 / (double)M;
+    for (int v = 0; v < EM_NV; ++v) s[v] *= inv;
+    double *o = obs + img * 3;
+    const double Q = o[2];
+    o[0] = s[EM_HW] - log(Q);
+    double res = s[EM_RA] / (s[EM_XA] + s[EM_BA]) + s[EM_RB] / (s[EM_XB] + s[EM_BB]);
+    if (compressible) res /= 2.0;
+    else res = (res + s[EM_RY] / (1.0 + s[EM_BY])) / 3.0;
+    o[1] = res;
+}
+
+// NBC by PBC (src/Model_AB.cc:148-182): the upper half of the mirrored axis
+// (x in 1-D, y in 2-D, z in 3-D) takes the lower half reversed; the
+// Boltzmann factors follow their fields.
+template <typename T>
+__global__ void k_str_mirror(double *wA, double *wB, T *eA2, T *eA4, T *eB2, T *eB4, const int *active, int dim,
+                             int Lx, int Ly, int Lz) {
+    const int img = blockIdx.y;
+    if (!active[img]) return;
+    const long M = (long)Lx * Ly * Lz, o = (long)img * M;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < M; i += (long)gridDim.x * blockDim.x) {
+        int x = (int)(i / ((long)Ly * Lz)), y = (int)((i / Lz) % Ly), z = (int)(i % Lz);
+        if (dim == 1) { if (x < Lx / 2) continue; x = Lx - 1 - x; }
+        else if (dim == 2) { if (y < Ly / 2) continue; y = Ly - 1 - y; }
+        else { if (z < Lz / 2) continue; z = Lz - 1 - z; }
+        const long s = o + ((long)x * Ly + y) * Lz + z, d = o + i;
+        wA[d] = wA[s];
+        wB[d] = wB[s];
+        eA2[d] = eA2[s];
+        eA4[d] = eA4[s];
+        eB2[d] = eB2[s];
+        eB4[d] = eB4[s];
+    }
+}
+
+// part[p][blk]: partial sums of (wA[p+1] - wA[p])^2 (arc length, scft/polystring_AB.cc:283-297)
+__global__ void __launch_bounds__(kStrThreads) k_str_arc(const double *__restrict__ w, long M, double *part) {
+    __shared__ double sh[kStrThreads / 64];
+    const double *a = w + (long)blockIdx.y * M, *b = a + M;
+    double s = 0.0;
+    for (long i = blockIdx.x * (long)kStrThreads + threadIdx.x; i < M; i += (long)gridDim.x * kStrThreads) {
+        const double d = b[i] - a[i];
+        s += d * d;
+    }
+    s = str_block_sum(s, sh);
+    if (threadIdx.x == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = s;
+}
+
+// Respline every grid point across the images: w[z] <- sum_j W[z][j] w[j]
+// for the inner images, the end images restored from their pinned values
+// (scft/polystring_AB.cc:299-329).
+__global__ void __launch_bounds__(kStrThreads) k_str_respline(double *w, const double *__restrict__ ends,
+                                                              const double *__restrict__ W, int m, long M) {
+    for (long i = blockIdx.x * (long)kStrThreads + threadIdx.x; i < M; i += (long)gridDim.x * kStrThreads) {
+        double y[kStrMaxImages];
+        for (int j = 0; j < m; ++j) y[j] = w[(long)j * M + i];
+        for (int z = 1; z + 1 < m; ++z) {
+            double s = 0.0;
+            for (int j = 0; j < m; ++j) s += W[z * m + j] * y[j];
+            w[(long)z * M + i] = s;
+        }
+        w[i] = ends[i];
+        w[(long)(m - 1) * M + i] = ends[M + i];
+    }
+}
+
+// Natural cubic spline through (S_j, y_j), evaluated at z/(m-1), as the
+// linear map y_new = W y (the zero-curvature ends of scft/spline.h):
+// second derivatives c from the tridiagonal system
+//   h_{i-1} c_{i-1} + 2 (h_{i-1} + h_i) c_i + h_i c_{i+1} = 6 (d_i - d_{i-1}),
+// c_0 = c_{m-1} = 0, d_i = (y_{i+1} - y_i) / h_i, then the cubic on the
+// interval holding each target.
+static void spline_weights(int m, const double *S, std::vector<double> &W) {
+    W.assign((size_t)m * m, 0.0);
+    std::vector<double> h(m), lo(m), di(m), up(m), r(m), c(m), y(m);
+    for (int i = 0; i + 1 < m; ++i) h[i] = S[i + 1] - S[i];
+    for (int col = 0; col < m; ++col) {
+        std::fill(y.begin(), y.end(), 0.0);
+        y[col] = 1.0;
+        std::fill(c.begin(), c.end(), 0.0);
+        const int n = m - 2;
+        for (int i = 1; i <= n; ++i) {
+            lo[i] = h[i - 1];
+            di[i] = 2.0 * (h[i - 1] + h[i]);
+            up[i] = h[i];
+            r[i] = 6.0 * ((y[i + 1] - y[i]) / h[i] - (y[i] - y[i - 1]) / h[i - 1]);
+        }
+        for (int i = 2; i <= n; ++i) {
+            const double f = lo[i] / di[i - 1];
+            di[i] -= f * up[i - 1];
+            r[i] -= f * r[i - 1];
+        }
+        if (n >= 1) {
+            c[n] = r[n] / di[n];
+            for (int i = n - 1; i >= 1; --i) c[i] = (r[i] - up[i] * c[i + 1]) / di[i];
+        }
+        for (int z = 0; z < m; ++z) {
+            const double x = (double)z / (m - 1);
+            int i = 0;
+            while (i < m - 2 && x > S[i + 1]) ++i;
+            const double p = S[i + 1] - x, q = x - S[i], hi = h[i];
+            W[(size_t)z * m + col] = (c[i] * p * p * p + c[i + 1] * q * q * q) / (6.0 * hi) +
+                                     (y[i] / hi - c[i] * hi / 6.0) * p + (y[i + 1] / hi - c[i + 1] * hi / 6.0) * q;
+        }
+    }
+}
+
+}  // namespace po
+
+struct po_string {
+    po_ctx *ctx = nullptr;
+    po_string_desc desc{};
+    virtual ~po_string() = default;
+    virtual int iterate(int max_iter, int min_iter, int record_interval, double thresh, double *H, double *S) = 0;
+    virtual int get(int which, double *out) = 0;
+    virtual long contour_steps() const = 0;
+};
+
+namespace po {
+
+template <typename T> struct StringAB : po_string {
+    Shape sh{};
+    Cell cell{};
+    Roots<T> rt{};
+    int m = 0, sA = 0, sB = 0, quadA = 0, quadB = 0;
+    double dsA = 0, dsB = 0;
+    DBuf<double> wA, wB, yita, phiA, phiB, ends, part, obs, W, arcs;
+    DBuf<int> active;
+    DBuf<T> eA2, eA4, eB2, eB4, qA, qB, qAc, qBc;
+    DBuf<cx<T>> spec;
+    KTable<T> kA, kAh, kB, kBh;
+    Program<T> prog;
+
+    int init(po_ctx *c, const po_string_desc &d, const double *hA, const double *hB) {
+        ctx = c;
+        desc = d;
+        int rc;
+        if ((rc = make_shape(d.grid, sh))) return rc;
+        if ((rc = make_cell(d.grid, cell))) return rc;
+        if ((rc = make_roots<T>(c, sh, rt))) return rc;
+        if (d.m < 3 || d.m > kStrMaxImages)
+            return fail(PO_ERR_INVALID, "a string needs 3 to " + std::to_string(kStrMaxImages) + " images");
+        if (d.sA < 2 || d.sB < 2) return fail(PO_ERR_INVALID, "Ms entries must be >= 2");
+        if (d.algo != PO_RQM4 && d.algo != PO_OS2) return fail(PO_ERR_INVALID, "algo must be RQM4 or OS2");
+        m = d.m;
+        sA = d.sA;
+        sB = d.sB;
+        const int LA = sA - 1, LB = sB - 1;  // src/Model_AB.cc:100-111
+        dsA = (1.0 * LA / (LA + LB)) / LA;
+        dsB = (1.0 * LB / (LA + LB)) / LB;
+        if (d.contour == PO_QUAD_TRAP) {
+            quadA = quadB = QUAD_TRAP;
+        } else {
+            quadA = sA % 2 != 0 ? QUAD_SIMPSON : QUAD_QUAD4C;
+            quadB = sB % 2 != 0 ? QUAD_SIMPSON : QUAD_QUAD4C;
+            if ((quadA == QUAD_SIMPSON && LA < 4) || (quadA == QUAD_QUAD4C && LA < 7) ||
+                (quadB == QUAD_SIMPSON && LB < 4) || (quadB == QUAD_QUAD4C && LB < 7))
+                return fail(PO_ERR_QUADRATURE, "too few contour slices for the quadrature");
+        }
+        const long M = sh.M;
+        const size_t mM = (size_t)m * M;
+        hipStream_t st = c->stream;
+        PO_HIP(wA.alloc(mM));
+        PO_HIP(wB.alloc(mM));
+        PO_HIP(yita.alloc(mM));
+        PO_HIP(phiA.alloc(mM));
+        PO_HIP(phiB.alloc(mM));
+        PO_HIP(ends.alloc(4 * (size_t)M));
+        PO_HIP(part.alloc((size_t)m * kStrBlocks * EM_NV));
+        PO_HIP(obs.alloc((size_t)m * 3));
+        PO_HIP(W.alloc((size_t)m * m));
+        PO_HIP(arcs.alloc(m));
+        PO_HIP(active.alloc(m));
+        PO_HIP(eA2.alloc(mM));
+        PO_HIP(eA4.alloc(mM));
+        PO_HIP(eB2.alloc(mM));
+        PO_HIP(eB4.alloc(mM));
+        PO_HIP(qA.alloc((size_t)sA * mM));
+        PO_HIP(qAc.alloc((size_t)sA * mM));
+        PO_HIP(qB.alloc((size_t)sB * mM));
+        PO_HIP(qBc.alloc((size_t)sB * mM));
+        PO_HIP(spec.alloc((size_t)4 * m * sh.Mh));
+        PO_HIP(hipMemcpyAsync(wA.p, hA, sizeof(double) * mM, hipMemcpyHostToDevice, st));
+        PO_HIP(hipMemcpyAsync(wB.p, hB, sizeof(double) * mM, hipMemcpyHostToDevice, st));
+        // pinned ends: images 0 and m-1 of the initial string (scft/polystring_AB.cc:236-244)
+        PO_HIP(hipMemcpyAsync(ends.p, hA, sizeof(double) * M, hipMemcpyHostToDevice, st));
+        PO_HIP(hipMemcpyAsync(ends.p + M, hA + (size_t)(m - 1) * M, sizeof(double) * M, hipMemcpyHostToDevice, st));
+        PO_HIP(hipMemcpyAsync(ends.p + 2 * M, hB, sizeof(double) * M, hipMemcpyHostToDevice, st));
+        PO_HIP(hipMemcpyAsync(ends.p + 3 * M, hB + (size_t)(m - 1) * M, sizeof(double) * M, hipMemcpyHostToDevice,
+                              st));
+        PO_HIP(launch_fill<double>(phiA.p, 0.0, (long)mM, st));
+        PO_HIP(launch_fill<double>(phiB.p, 0.0, (long)mM, st));
+        PO_HIP(launch_fill<double>(obs.p, 0.0, (long)m * 3, st));
+        if ((rc = factors())) return rc;
+        const bool general = (d.algo == PO_OS2) && !cell.ortho;
+        if ((rc = build_ktable<T>(kA, sh, cell, general, dsA, st))) return rc;
+        if ((rc = build_ktable<T>(kB, sh, cell, general, dsB, st))) return rc;
+        if (d.algo == PO_RQM4) {
+            if ((rc = build_ktable<T>(kAh, sh, cell, false, 0.5 * dsA, st))) return rc;
+            if ((rc = build_ktable<T>(kBh, sh, cell, false, 0.5 * dsB, st))) return rc;
+        }
+        // lanes 2i: qA_i -> qB_i, 2i+1: qBc_i -> qAc_i (src/Model_AB.cc:139-145)
+        std::vector<Lane<T>> lanes(2 * m);
+        for (int i = 0; i < m; ++i) {
+            const size_t o = (size_t)i * M;
+            T *a = qA.p + (size_t)i * sA * M, *b = qB.p + (size_t)i * sB * M;
+            T *ac = qAc.p + (size_t)i * sA * M, *bc = qBc.p + (size_t)i * sB * M;
+            lanes[2 * i].segs = {Segment<T>{a, sA, eA2.p + o, eA4.p + o, &kA, &kAh},
+                                 Segment<T>{b, sB, eB2.p + o, eB4.p + o, &kB, &kBh}};
+            lanes[2 * i + 1].segs = {Segment<T>{bc, sB, eB2.p + o, eB4.p + o, &kB, &kBh},
+                                     Segment<T>{ac, sA, eA2.p + o, eA4.p + o, &kA, &kAh}};
+            for (int l = 0; l < 2; ++l) {
+                lanes[2 * i + l].Sf = spec.p + (size_t)(4 * i + 2 * l) * sh.Mh;
+                lanes[2 * i + l].Sh = spec.p + (size_t)(4 * i + 2 * l + 1) * sh.Mh;
+            }
+            PO_HIP(launch_fill<T>(a, (T)1, M, st));   // heads qA[0] = qBc[0] = 1
+            PO_HIP(launch_fill<T>(bc, (T)1, M, st));  // (src/Model_AB.cc:779-782)
+        }
+        prog.slice_elems = M;
+        if ((rc = prog.build(d.algo, lanes, st))) return rc;
+        return prog.tune(sh, rt, st);
+    }
+
+    int factors() {
+        hipStream_t st = ctx->stream;
+        const long n = (long)m * sh.M;
+        PO_HIP(launch_exp_factor<T>(wA.p, eA2.p, -0.5 * dsA, n, st));
+        PO_HIP(launch_exp_factor<T>(wA.p, eA4.p, -0.25 * dsA, n, st));
+        PO_HIP(launch_exp_factor<T>(wB.p, eB2.p, -0.5 * dsB, n, st));
+        PO_HIP(launch_exp_factor<T>(wB.p, eB4.p, -0.25 * dsB, n, st));
+        return PO_OK;
+    }
+
+    // one EM update of every active image (src/Model_AB.cc:136-261)
+    int update_once(const std::vector<int> &act) {
+        hipStream_t st = ctx->stream;
+        const long M = sh.M;
+        if (desc.mirror) {
+            hipLaunchKernelGGL(k_str_mirror<T>, dim3(kStrBlocks, m), dim3(kStrThreads), 0, st, wA.p, wB.p, eA2.p,
+                               eA4.p, eB2.p, eB4.p, active.p, sh_dim(), sh.Lx, sh.Ly, sh.Lz);
+            PO_HIP(hipGetLastError());
+        }
+        int rc = prog.run(sh, rt, st);
+        if (rc) return rc;
+        // Q of every image: qB->Qt() (src/Model_AB.cc:195)
+        hipLaunchKernelGGL(k_str_tail<T>, dim3(kStrBlocks, m), dim3(kStrThreads), 0, st,
+                           qB.p + (size_t)(sB - 1) * M, (long)sB * M, M, part.p);
+        PO_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_str_rows, dim3((m + 63) / 64), dim3(64), 0, st, part.p, m, kStrBlocks, (double)M,
+                           obs.p + 2, 3);
+        PO_HIP(hipGetLastError());
+        for (int i = 0; i < m; ++i) {
+            if (!act[i]) continue;
+            const size_t o = (size_t)i * M;
+            PO_HIP(launch_density<T>(qA.p + (size_t)i * sA * M, qAc.p + (size_t)i * sA * M, sA, M, quadA,
+                                     obs.p + 3 * i + 2, 0.0, dsA, phiA.p + o, st));
+            PO_HIP(launch_density<T>(qB.p + (size_t)i * sB * M, qBc.p + (size_t)i * sB * M, sB, M, quadB,
+                                     obs.p + 3 * i + 2, 0.0, dsB, phiB.p + o, st));
+        }
+        EmArgs a{};
+        a.wA = wA.p;
+        a.wB = wB.p;
+        a.yita = yita.p;
+        a.phiA = phiA.p;
+        a.phiB = phiB.p;
+        a.active = active.p;
+        a.M = M;
+        a.chiN = desc.chiN;
+        a.lamA = desc.lamA;
+        a.lamB = desc.lamB;
+        a.lamY = desc.lamY;
+        a.compressible = desc.compressible;
+        a.cA2 = -0.5 * dsA;
+        a.cA4 = -0.25 * dsA;
+        a.cB2 = -0.5 * dsB;
+        a.cB4 = -0.25 * dsB;
+        a.part = part.p;
+        hipLaunchKernelGGL(k_str_em<T>, dim3(kStrBlocks, m), dim3(kStrThreads), 0, st, a, eA2.p, eA4.p, eB2.p,
+                           eB4.p);
+        PO_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_str_obs, dim3((m + 63) / 64), dim3(64), 0, st, part.p, kStrBlocks, M, active.p,
+                           desc.compressible, obs.p, m);
+        PO_HIP(hipGetLastError());
+        return PO_OK;
+    }
+
+    int sh_dim() const { return desc.grid.dim; }
+
+    int iterate(int max_iter, int min_iter, int record_interval, double thresh, double *H, double *S) override {
+        if (max_iter < 1 || record_interval < 1) return fail(PO_ERR_INVALID, "bad relax parameters");
+        hipStream_t st = ctx->stream;
+        const long M = sh.M;
+        // every image restarts from its fields with a fresh yita (Model_AB::resetInString)
+        if (!desc.compressible) PO_HIP(launch_fill<double>(yita.p, 0.0, (long)m * M, st));
+        std::vector<int> act(m, 1), nrec(m, 0);
+        std::vector<double> o(3 * (size_t)m);
+        PO_HIP(hipMemcpyAsync(active.p, act.data(), sizeof(int) * m, hipMemcpyHostToDevice, st));
+        // scft::relax per image (src/scft.cc:382-422, stopping rules :536-568)
+        for (int it = 1; it <= max_iter; ++it) {
+            int rc = update_once(act);
+            if (rc) return rc;
+            if (it % record_interval != 0) continue;
+            PO_HIP(hipMemcpyAsync(o.data(), obs.p, sizeof(double) * 3 * m, hipMemcpyDeviceToHost, st));
+            PO_HIP(hipStreamSynchronize(st));
+            bool changed = false, any = false;
+            for (int i = 0; i < m; ++i) {
+                if (!act[i]) continue;
+                ++nrec[i];
+                const bool stop = !std::isfinite(o[3 * i]) || (it > min_iter && nrec[i] > 1 && o[3 * i + 1] < thresh);
+                if (stop) {
+                    act[i] = 0;
+                    changed = true;
+                } else {
+                    any = true;
+                }
+            }
+            if (!any) break;
+            if (changed) PO_HIP(hipMemcpyAsync(active.p, act.data(), sizeof(int) * m, hipMemcpyHostToDevice, st));
+        }
+        // images that ended between records keep the H of their last update
+        PO_HIP(hipMemcpyAsync(o.data(), obs.p, sizeof(double) * 3 * m, hipMemcpyDeviceToHost, st));
+        PO_HIP(hipStreamSynchronize(st));
+        for (int i = 0; i < m; ++i) H[i] = o[3 * i];
+        // arc length of wA along the string (scft/polystring_AB.cc:283-297)
+        hipLaunchKernelGGL(k_str_arc, dim3(kStrBlocks, m - 1), dim3(kStrThreads), 0, st, wA.p, M, part.p);
+        PO_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_str_rows, dim3((m + 63) / 64), dim3(64), 0, st, part.p, m - 1, kStrBlocks, (double)M,
+                           arcs.p, 1);
+        PO_HIP(hipGetLastError());
+        std::vector<double> arc(m);
+        PO_HIP(hipMemcpyAsync(arc.data(), arcs.p, sizeof(double) * (m - 1), hipMemcpyDeviceToHost, st));
+        PO_HIP(hipStreamSynchronize(st));
+        std::vector<double> s(m, 0.0);
+        for (int i = 1; i < m; ++i) s[i] = s[i - 1] + std::sqrt(arc[i - 1]);
+        for (int i = 1; i < m; ++i) s[i] /= s[m - 1];
+        if (S) for (int i = 0; i < m; ++i) S[i] = s[i];
+        // redistribute the images at equal arc length, ends pinned (:299-329)
+        std::vector<double> Wh;
+        spline_weights(m, s.data(), Wh);
+        PO_HIP(hipMemcpyAsync(W.p, Wh.data(), sizeof(double) * m * m, hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_str_respline, dim3(kStrBlocks * 4), dim3(kStrThreads), 0, st, wA.p, ends.p, W.p, m, M);
+        PO_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_str_respline, dim3(kStrBlocks * 4), dim3(kStrThreads), 0, st, wB.p, ends.p + 2 * M, W.p,
+                           m, M);
+        PO_HIP(hipGetLastError());
+        int rc = factors();
+        if (rc) return rc;
+        PO_HIP(hipStreamSynchronize(st));
+        return PO_OK;
+    }
+
+    int get(int which, double *out) override {
+        const double *src = which == PO_FIELD_WA ? wA.p : which == PO_FIELD_WB ? wB.p
+                          : which == PO_FIELD_PHIA ? phiA.p : which == PO_FIELD_PHIB ? phiB.p : nullptr;
+        if (!src) return fail(PO_ERR_INVALID, "unknown field");
+        PO_HIP(hipMemcpyAsync(out, src, sizeof(double) * (size_t)m * sh.M, hipMemcpyDeviceToHost, ctx->stream));
+        PO_HIP(hipStreamSynchronize(ctx->stream));
+        return PO_OK;
+    }
+
+    long contour_steps() const override { return 2L * m * ((sA - 1) + (sB - 1)); }
+};
+
+}  // namespace po
+
+using po::fail;
+
+extern "C" {
+
+int po_string_create(po_ctx *ctx, const po_string_desc *desc, const double *wA, const double *wB, po_string **out) {
+    if (!ctx || !desc || !wA || !wB || !out) return fail(PO_ERR_INVALID, "null argument");
+    PO_HIP(hipSetDevice(ctx->device));
+    po_string *s;
+    int rc;
+    if (ctx->precision == PO_FP64) {
+        auto *t = new po::StringAB<double>();
+        rc = t->init(ctx, *desc, wA, wB);
+        s = t;
+    } else {
+        auto *t = new po::StringAB<float>();
+        rc = t->init(ctx, *desc, wA, wB);
+        s = t;
+    }
+    if (rc) {
+        delete s;
+        return rc;
+    }
+    *out = s;
+    return PO_OK;
+}
+
+int po_string_destroy(po_string *s) {
+    delete s;
+    return PO_OK;
+}
+
+int po_string_iterate(po_string *s, int max_iter, int min_iter, int record_interval, double thresh, double *H,
+                      double *S) {
+    if (!s || !H) return fail(PO_ERR_INVALID, "null argument");
+    PO_HIP(hipSetDevice(s->ctx->device));
+    return s->iterate(max_iter, min_iter, record_interval, thresh, H, S);
+}
+
+int po_string_get(po_string *s, int which, double *host_out) {
+    if (!s || !host_out) return fail(PO_ERR_INVALID, "null argument");
+    return s->get(which, host_out);
+}
+
+long po_string_contour_steps(const po_string *s) { return s ? s->contour_steps() : -1; }
+
+}  // extern "C"

@@ -217,3 +217,81 @@ def test_gauge_invariance_and_mean_density():
     assert abs(m1.H() - m2.H()) < 1e-11 * abs(m1.H())
     assert abs(m1.get("phiA").mean() - 0.3) < 1e-13
     assert abs(m1.get("phiB").mean() - 0.7) < 1e-13
+
+
+# ------------------------------------------------------------ string method
+def test_em_lam1d_published_free_energy():
+    """Simple mixing (EM) on the 1-D LAM example converges to the published
+    RQM4+EM H = 4.778413059823560 (residual 1e-8, lam = [0.05, 0.05, 5])."""
+    known = json.load(open(os.path.join(GOLD, "known_answers.json")))["1D"]
+    f = np.load(os.path.join(GOLD, "lam1d_field.npz"))
+    m = O.EMModel(f["wA"], f["wB"], dim=1, cell=("LAMELLAR", 4.0), Ms=(51, 51), chiN=30.0, lam=(0.05, 0.05, 5.0))
+    it, H, res = m.relax(20000, 10, 1, 1e-8)
+    assert res < 1e-8
+    assert abs(H - known["RQM4+EM"]["H"]) < 1e-8
+    # the incompressibility multiplier has done its job: <phiA + phiB> = 1
+    assert abs((m.get("phiA") + m.get("phiB")).mean() - 1.0) < 1e-12
+
+
+def test_em_compressible_matches_penalty_functional():
+    """Compressible EM: at a fixed point w = chiN phi_other + lamY (phiA + phiB - 1), so
+    the residual vanishes and H includes the penalty lamY/2 <(phiA+phiB-1)^2>."""
+    f = np.load(os.path.join(GOLD, "lam1d_field.npz"))
+    m = O.EMModel(f["wA"], f["wB"], dim=1, cell=("LAMELLAR", 4.0), Ms=(51, 51), chiN=30.0, lam=(0.05, 0.05, 50.0),
+                  compressible=True)
+    it, H, res = m.relax(20000, 10, 1, 1e-10)
+    assert res < 1e-10
+    pA, pB, wA, wB = m.get("phiA"), m.get("phiB"), m.get("wA"), m.get("wB")
+    eta = 50.0 * (pA + pB - 1.0)
+    assert np.abs(wA - (30.0 * pB + eta)).max() < 1e-7
+    Hw = np.mean(30.0 * pA * pB + 25.0 * (pA + pB - 1.0) ** 2 - wA * pA - wB * pB)
+    assert abs(Hw - m.Hw()) < 1e-12
+
+
+def test_em_mirror_makes_fields_symmetric():
+    """NBC by PBC: after an update the mirrored axis is reflection-symmetric
+    (src/Model_AB.cc:148-182), and stays so."""
+    rng = np.random.default_rng(3)
+    for dim, shape, cell in ((1, (32, 1, 1), ("LAMELLAR", 4.0)), (2, (8, 16, 1), ("RECTANGULAR", 3.0, 4.0)),
+                             (3, (4, 4, 8), ("ORTHORHOMBIC", 3.0, 3.2, 4.0))):
+        wA = 10.0 + rng.standard_normal(shape)
+        wB = 10.0 + rng.standard_normal(shape)
+        m = O.EMModel(wA, wB, dim=dim, cell=cell, Ms=(11, 11), chiN=20.0, lam=(0.1, 0.1, 1.0), mirror=True)
+        for _ in range(2):
+            m.update()
+        ax = dim - 1
+        for name in ("wA", "phiA"):
+            x = m.get(name)
+            n = shape[ax]
+            lower = np.take(x, range(n // 2), axis=ax)
+            upper = np.take(x, range(n // 2, n), axis=ax)
+            assert np.abs(upper - np.flip(lower, axis=ax)).max() < 1e-12, (dim, name)
+
+
+def test_spline_weights_match_scipy_natural_cubic_spline():
+    from scipy.interpolate import CubicSpline
+    rng = np.random.default_rng(5)
+    for m in (3, 4, 7, 32, 61):
+        S = np.concatenate([[0.0], np.sort(rng.uniform(0, 1, m - 2)), [1.0]])
+        W = O.spline_weights(S)
+        Y = rng.standard_normal((m, 6))
+        ref = CubicSpline(S, Y, bc_type="natural")(np.arange(m) / (m - 1))
+        assert np.abs(W @ Y - ref).max() < 1e-12 * max(1.0, np.abs(Y).max())
+        # a straight line is reproduced exactly (the spline is exact on linear data)
+        assert np.abs(W @ (2.0 * S + 1.0) - (2.0 * np.arange(m) / (m - 1) + 1.0)).max() < 1e-12
+
+
+def test_string_arclength_and_equal_spacing():
+    rng = np.random.default_rng(6)
+    wa = rng.standard_normal((6, 4, 4, 4))
+    S = O.string_arclength(wa)
+    d = np.sqrt(((wa[1:] - wa[:-1]) ** 2).reshape(5, -1).mean(1))
+    ref = np.concatenate([[0.0], np.cumsum(d)]) / d.sum()
+    assert np.abs(S - ref).max() < 1e-15
+    # images already equally spaced along a straight line stay where they are
+    t = np.linspace(0, 1, 6).reshape(6, 1, 1, 1)
+    line = wa[0] + t * (wa[-1] - wa[0])
+    S = O.string_arclength(line)
+    assert np.abs(S - np.linspace(0, 1, 6)).max() < 1e-14
+    W = O.spline_weights(S)
+    assert np.abs((W @ line.reshape(6, -1)) - line.reshape(6, -1)).max() < 1e-12

@@ -181,6 +181,18 @@ int po_model_set_cell(po_model *m, double a, double b, double c, double alpha, d
 int po_model_relax_cell(po_model *m, int max_outer, double stress_tol, int max_iter, int min_iter,
                         int record_interval, double thresh, double *cell6, double *H, double *stress3, int *outer);
 
+/* ------------------------------------------------------------ ragged scan */
+/* Replay the propagator chains of every update from a hipGraph (on != 0),
+ * captured at the next update; no reference counterpart. */
+int po_model_set_graph(po_model *m, int on);
+/* po_model_relax over n independent models at once (a phase scan of uneven
+ * grids and contour counts), each with the reference's stopping rules
+ * (src/scft.cc:382-422,536-568).  Updates are issued round-robin to each
+ * model's own context stream, so models on different contexts overlap on the
+ * device.  iters, H, residual (optional) get n values. */
+int po_models_relax(po_model **models, int n, int max_iter, int min_iter, int record_interval, double thresh,
+                    int *iters, double *H, double *residual);
+
 /* ---------------------------------------------------------- string method */
 /* The string driver scft/polystring_AB.cc with every image on the GPU at once:
  * m images of one AB melt, each iterated by Model_AB's simple mixing (EM)

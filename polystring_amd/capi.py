@@ -34,7 +34,7 @@ EXPORTS = [
     "po_model_relax", "po_model_contour_steps", "po_model_stress", "po_model_set_cell",
     "po_model_relax_cell", "po_model_set_adjoint", "po_model_last_stress", "po_model_grad_w",
     "po_model_set_fields", "po_string_create", "po_string_destroy", "po_string_iterate", "po_string_get",
-    "po_string_contour_steps",
+    "po_string_contour_steps", "po_model_set_graph", "po_models_relax",
     "po_model_set_profiling", "po_model_profile",
 ]
 
@@ -100,6 +100,8 @@ def load():
         "po_string_create": ([_vp, ctypes.POINTER(StringDesc), _dp, _dp, ctypes.POINTER(_vp)], i),
         "po_string_destroy": ([_vp], i), "po_string_iterate": ([_vp, i, i, i, d, _dp, _dp], i),
         "po_string_get": ([_vp, i, _dp], i), "po_string_contour_steps": ([_vp], ctypes.c_long),
+        "po_model_set_graph": ([_vp, i], i),
+        "po_models_relax": ([ctypes.POINTER(_vp), i, i, i, i, d, ctypes.POINTER(i), _dp, _dp], i),
         "po_model_set_profiling": ([_vp, i], i), "po_model_profile": ([_vp, _dp], i),
     }
     for name, (args, res) in sig.items():
@@ -287,6 +289,9 @@ class ModelAB:
         wB = np.ascontiguousarray(wB, dtype=np.float64).reshape(self.shape)
         _check(load().po_model_set_fields(self.h, _p(wA), _p(wB)))
 
+    def set_graph(self, on=True):
+        _check(load().po_model_set_graph(self.h, 1 if on else 0))
+
     def set_adjoint(self, on=True):
         """Fused per-update stress (see last_stress)."""
         _check(load().po_model_set_adjoint(self.h, 1 if on else 0))
@@ -308,6 +313,18 @@ class ModelAB:
         o = np.zeros(4)
         _check(load().po_model_profile(self.h, _p(o)))
         return dict(prop_ms=o[0], rest_ms=o[1], iters=int(o[2]), lockstep_steps=int(o[3]))
+
+
+def relax_models(models, max_iter, min_iter=10, record_interval=1, thresh=1e-8):
+    """po_models_relax: relax independent models concurrently; returns (iters, H, residual) arrays."""
+    n = len(models)
+    hs = (_vp * n)(*[m.h for m in models])
+    it = (ctypes.c_int * n)()
+    H = np.zeros(n)
+    r = np.zeros(n)
+    _check(load().po_models_relax(hs, n, int(max_iter), int(min_iter), int(record_interval), float(thresh), it,
+                                  _p(H), _p(r)))
+    return np.array(list(it)), H, r
 
 
 class String:

@@ -153,6 +153,7 @@ struct po_model {
     virtual int observe(double *out) = 0;
     virtual int get(int which, double *out) = 0;
     virtual int stress(double *dHdl, double *Q) = 0;
+    virtual void set_graph(bool on) = 0;
     virtual int set_adjoint(bool on) = 0;
     virtual int grad_w(double *gA, double *gB, double *Q) = 0;
     virtual int set_fields(const double *wA, const double *wB) = 0;
@@ -389,6 +390,11 @@ template <typename T> struct Model : po_model {
         prog_adj.plane_variant = prog.plane_variant;
         fused_on = true;
         return PO_OK;
+    }
+
+    void set_graph(bool on) override {
+        prog.use_graph = on ? 1 : 0;
+        prog_adj.use_graph = on ? 1 : 0;
     }
 
     int last_stress(double *dHdl) override {
@@ -1053,6 +1059,59 @@ int po_model_set_fields(po_model *m, const double *wA, const double *wB) {
     if (!m || !wA || !wB) return fail(PO_ERR_INVALID, "null argument");
     PO_HIP(hipSetDevice(m->ctx->device));
     return m->set_fields(wA, wB);
+}
+
+int po_model_set_graph(po_model *m, int on) {
+    if (!m) return fail(PO_ERR_INVALID, "null model");
+    PO_HIP(hipStreamSynchronize(m->ctx->stream));
+    m->set_graph(on != 0);
+    return PO_OK;
+}
+
+int po_models_relax(po_model **ms, int n, int max_iter, int min_iter, int record_interval, double thresh,
+                    int *iters, double *H, double *residual) {
+    if (!ms || n < 1 || record_interval < 1) return fail(PO_ERR_INVALID, "bad argument");
+    for (int i = 0; i < n; ++i)
+        if (!ms[i]) return fail(PO_ERR_INVALID, "null model in the list");
+    std::vector<int> it(n, 0), nrec(n, 0), act(n, 1);
+    std::vector<double> h(n, 0.0), r(n, 0.0);
+    int left = n;
+    // round-robin: each model's updates go to its own context's stream, so
+    // models on different contexts run concurrently on the device
+    while (left > 0) {
+        for (int i = 0; i < n; ++i) {
+            if (!act[i]) continue;
+            const int chunk = std::min(record_interval - it[i] % record_interval, max_iter - it[i]);
+            PO_HIP(hipSetDevice(ms[i]->ctx->device));
+            int rc = ms[i]->update(chunk);
+            if (rc) return rc;
+            it[i] += chunk;
+        }
+        for (int i = 0; i < n; ++i) {
+            if (!act[i]) continue;
+            bool stop = it[i] >= max_iter;
+            if (it[i] % record_interval == 0) {
+                double o[5];
+                int rc = ms[i]->observe(o);
+                if (rc) return rc;
+                ++nrec[i];
+                h[i] = o[0];
+                r[i] = o[3];
+                if (!std::isfinite(o[0])) stop = true;                              // src/scft.cc:537-543
+                if (it[i] > min_iter && nrec[i] > 1 && o[3] < thresh) stop = true;  // src/scft.cc:560-568
+            }
+            if (stop) {
+                act[i] = 0;
+                --left;
+            }
+        }
+    }
+    for (int i = 0; i < n; ++i) {
+        if (iters) iters[i] = it[i];
+        if (H) H[i] = h[i];
+        if (residual) residual[i] = r[i];
+    }
+    return PO_OK;
 }
 
 int po_model_set_adjoint(po_model *m, int on) {

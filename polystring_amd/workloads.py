@@ -100,6 +100,69 @@ def cfg4(L=64):
     return Workload("cfg4", shape, ("ORTHORHOMBIC",) + lengths, 200, f, chiN, wA, wB)
 
 
+@dataclass
+class StringWorkload:
+    name: str
+    shape: tuple
+    cell: tuple
+    Ns: int
+    f: float
+    chiN: float
+    wA: np.ndarray = field(repr=False)   # (m, Lx, Ly, Lz)
+    wB: np.ndarray = field(repr=False)
+    lam: tuple = (0.1, 0.1, 1.0)
+    respline_every: int = 10
+
+    @property
+    def Ms(self):
+        return ms_from(self.f, self.Ns)
+
+    @property
+    def m(self):
+        return self.wA.shape[0]
+
+    def string_kwargs(self):
+        return dict(cell=self.cell, Ms=self.Ms, chiN=self.chiN, lam=self.lam)
+
+
+def cfg5a(m=32, L=64, Ns=150):
+    """String of m images, AB melt f=0.33, chiN=17.5, 64^3, Ns=150: ends are
+    lamellar and hexagonal-cylinder templates, the inner images their linear
+    interpolation, plus noise sigma 0.05 (seed 1005, streams 0 / 1 per field,
+    2k / 2k+1 per image)."""
+    shape = (L, L, L)
+    f, chiN, a = 0.33, 17.5, 4.0
+    lengths = (a, a, a)
+    ends = [lamellar(shape, lengths), hex_cylinders(shape, lengths)]
+    t = np.linspace(0.0, 1.0, m)
+    wA = np.empty((m,) + shape)
+    wB = np.empty((m,) + shape)
+    for k in range(m):
+        g = (1 - t[k]) * ends[0] + t[k] * ends[1]
+        wA[k] = chiN * (1 - f) + 2 * g + noise(1005, 2 * k, shape, 0.05)
+        wB[k] = chiN * f - 2 * g + noise(1005, 2 * k + 1, shape, 0.05)
+    return StringWorkload("cfg5a", shape, ("CUBIC", a, a, a), Ns, f, chiN, wA, wB)
+
+
+def cfg5b(n=24, sizes=(32, 48, 64, 96), Ns_choices=(100, 150, 200)):
+    """Ragged scan: n independent solves with grids, Ns, f ~ U[.25, .45] and
+    chiN ~ U[14, 22] drawn from seed 1005 (stream 99), each started from a
+    gyroid template plus noise sigma 0.2 on a cubic cell a = 4.0."""
+    g = np.random.Generator(np.random.Philox(np.random.SeedSequence([1005, 99])))
+    out = []
+    for k in range(n):
+        L = int(sizes[g.integers(len(sizes))])
+        Ns = int(Ns_choices[g.integers(len(Ns_choices))])
+        f = float(g.uniform(0.25, 0.45))
+        chiN = float(g.uniform(14.0, 22.0))
+        shape, a = (L, L, L), 4.0
+        gy = gyroid(shape, (a, a, a))
+        wA = chiN * (1 - f) + 2 * gy + noise(1005, 100 + 2 * k, shape, 0.2)
+        wB = chiN * f - 2 * gy + noise(1005, 101 + 2 * k, shape, 0.2)
+        out.append(Workload(f"cfg5b-{k}", shape, ("CUBIC", a, a, a), Ns, f, chiN, wA, wB))
+    return out
+
+
 def contour_steps_per_iteration(Ns):
     """Counted contour steps per SCFT iteration: qA, qB, qBc, qAc = 2 Ns."""
     return 2 * Ns

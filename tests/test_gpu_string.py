@@ -81,3 +81,26 @@ def test_string_relax_stops_per_image(ctx64):
     Ho, So, owa, owb, _, _ = O.string_iterate(models, wa, wb, ends, relax)
     assert relerr(H, Ho) <= 1e-10
     assert relerr(g.get("wA"), owa) <= 1e-10
+
+
+def test_ragged_scan_matches_independent_runs(ctx64):
+    """po_models_relax over models of uneven grid and contour length on
+    separate contexts (streams), replayed from hipGraphs, gives each model
+    exactly what relaxing it alone gives."""
+    from polystring_amd import workloads
+    wls = [w for w in workloads.cfg5b(n=24) if w.shape[0] in (32, 48)][:4]
+    wls = [workloads.Workload(w.name, w.shape, w.cell, min(w.Ns, 60), w.f, w.chiN, w.wA, w.wB, n_anderson=5)
+           for w in wls]
+    ctxs = [capi.Context(0, "fp64") for _ in wls]
+    scan = []
+    for c, w in zip(ctxs, wls):
+        mdl = capi.ModelAB(c, w.wA, w.wB, **w.model_kwargs())
+        mdl.set_graph(True)
+        scan.append(mdl)
+    it, H, r = capi.relax_models(scan, max_iter=6, min_iter=2, record_interval=2, thresh=1e-12)
+    assert list(it) == [6] * len(wls)
+    for k, w in enumerate(wls):
+        alone = capi.ModelAB(ctx64, w.wA, w.wB, **w.model_kwargs())
+        it1, H1, r1 = alone.relax(6, 2, 2, 1e-12)
+        print(f"  {w.name} {w.shape} Ns={w.Ns}: scan H {H[k]!r} alone {H1!r}", flush=True)
+        assert it1 == 6 and H[k] == H1 and r[k] == r1

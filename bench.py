@@ -90,6 +90,56 @@ def run_form(precision, steps, warmup, adjoint, L=128, Ns=200):
     return out, first
 
 
+def run_string(iters=2, precision="fp32"):
+    """The batched string method on the 32-image 64^3 string (f=0.33,
+    chiN=17.5, Ns=150): per string iteration every image takes 10 EM updates,
+    then the arc-length respline."""
+    from polystring_amd import capi, workloads
+    wl = workloads.cfg5a()
+    ctx = capi.Context(0, precision)
+    s = capi.String(ctx, wl.wA, wl.wB, **wl.string_kwargs())
+    k = wl.respline_every
+    s.iterate(k, k, k, 0.0)  # warm-up string iteration
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        H, S = s.iterate(k, k, k, 0.0)
+    wall = time.perf_counter() - t0
+    steps = iters * k * s.contour_steps()
+    out = dict(workload=f"string method, {wl.m} images 64^3, Ns={wl.Ns}, f={wl.f}, chiN={wl.chiN}, "
+                        f"{k} EM updates per respline, {precision}",
+               string_iterations=iters, contour_steps_per_s=steps / wall, string_iterations_per_s=iters / wall,
+               H_mean=float(sum(H) / len(H)))
+    del s
+    ctx.close()
+    return out
+
+
+def run_scan(iters=5, precision="fp32"):
+    """Ragged scan: 24 independent solves of uneven grid (32..96^3) and
+    contour count (Ns 100..200), each on its own stream, propagators replayed
+    from hipGraphs, relaxed together by po_models_relax."""
+    from polystring_amd import capi, workloads
+    wls = workloads.cfg5b()
+    ctxs, models = [], []
+    for w in wls:
+        c = capi.Context(0, precision)
+        mdl = capi.ModelAB(c, w.wA, w.wB, **w.model_kwargs())
+        mdl.set_graph(True)
+        ctxs.append(c)
+        models.append(mdl)
+    capi.relax_models(models, 1, 1, 1, 0.0)  # warm-up + graph capture
+    t0 = time.perf_counter()
+    it, H, r = capi.relax_models(models, iters, iters, iters, 0.0)
+    wall = time.perf_counter() - t0
+    steps = sum(int(n) * m.contour_steps() for n, m in zip(it, models))
+    out = dict(workload=f"ragged scan, {len(wls)} solves, grids 32-96^3, Ns 100-200, {precision}",
+               iterations_each=iters, contour_steps_per_s=steps / wall, solves=len(wls))
+    del models
+    for c in ctxs:
+        c.close()
+    return out
+
+
 def cpu_baseline(seconds_target=15.0):
     """The FP64 oracle (a C restatement of the reference's RQM4 dataflow, C2C
     transforms) timed on this host: RQM4 contour steps of one propagator at
@@ -139,6 +189,7 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip the string and ragged-scan lines")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
@@ -163,6 +214,12 @@ def main():
         with open(PROFILE_TRAFFIC) as fh:
             key = "hbm_bytes_per_lockstep_step_" + ("fp32_adjoint" if head is mixed else "fp64")
             traffic = json.load(fh).get(key)
+    extras = {}
+    if not args.no_extras and args.form == "both":
+        extras["string_batched"] = run_string()
+        print(f"[bench] string {json.dumps(extras['string_batched'])}", file=sys.stderr, flush=True)
+        extras["ragged_scan"] = run_scan()
+        print(f"[bench] scan {json.dumps(extras['ragged_scan'])}", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu_baseline else cpu_baseline()
 
     line = {
@@ -211,6 +268,7 @@ def main():
             "residual_fp32": first32["residual"], "residual_fp64": first64["residual"],
         }
         line["trajectory_dH_over_H_after_run"] = abs(mixed["H"] - f64["H"]) / abs(f64["H"])
+    line.update(extras)
     print(json.dumps(line), flush=True)
 
 

@@ -251,38 +251,59 @@ k_anderson_d(AndersonArgs a, const double *__restrict__ phiA, const double *__re
 __global__ void k_anderson_solve(AndersonArgs a, const double *__restrict__ partials, int nblk,
                                  double *__restrict__ BA, double *__restrict__ BB, double *__restrict__ C,
                                  double *__restrict__ obs) {
+    constexpr int NB = (kMaxHist + 1) * (kMaxHist + 1);
     __shared__ double sums[2 * (kMaxHist + 1) + 2];
+    __shared__ double Bo[2][NB], Bn[2][NB];
     __shared__ double U[kMaxHist * kMaxHist], V[kMaxHist];
+    const int tid = threadIdx.x, nt = blockDim.x;
     const int nv = 2 * (a.nn + 1) + 2;
-    const int n1 = a.n + 1;
-    for (int v = threadIdx.x; v < nv; v += blockDim.x) {
+    const int n1 = a.n + 1, nn = a.nn;
+    // one wave per value: lanes take strided blocks, then a fixed shuffle tree
+    const int lane = tid & 63;
+    for (int v = tid >> 6; v < nv; v += nt >> 6) {
         double s = 0.0;
-        for (int b = 0; b < nblk; ++b) s += partials[(long)b * nv + v];
-        sums[v] = s / (double)a.M;
+        for (int b = lane; b < nblk; b += 64) s += partials[(long)b * nv + v];
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+        if (lane == 0) sums[v] = s / (double)a.M;
+    }
+    for (int idx = tid; idx < n1 * n1; idx += nt) {
+        Bo[0][idx] = BA[idx];
+        Bo[1][idx] = BB[idx];
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    // B(i,j) <- B(i-1,j-1) on the upper triangle, B(0,j) <- <d_k d_{k-j}> for
+    // j <= nn, then the lower triangle mirrors the upper (symmatu)
+    // (src/FieldAX.cc:188-199,333-345)
+    for (int idx = tid; idx < n1 * n1; idx += nt) {
+        const int i = idx / n1, j = idx % n1;
+        const int r = i <= j ? i : j, c = i <= j ? j : i;  // upper-triangle source
         for (int sp = 0; sp < 2; ++sp) {
-            double *B = sp ? BB : BA;
-            for (int i = a.n; i >= 1; --i)
-                for (int j = a.n; j >= i; --j) B[i * n1 + j] = B[(i - 1) * n1 + (j - 1)];
-            for (int j = 0; j <= a.nn; ++j) B[j] = sums[sp * (a.nn + 1) + j];
-            for (int i = 0; i <= a.n; ++i)
-                for (int j = i + 1; j <= a.n; ++j) B[j * n1 + i] = B[i * n1 + j];
+            double v;
+            if (r == 0) v = c <= nn ? sums[sp * (nn + 1) + c] : Bo[sp][c];
+            else v = Bo[sp][(r - 1) * n1 + (c - 1)];
+            Bn[sp][idx] = v;
         }
-        const int nn = a.nn;
-        for (int i = 0; i < nn; ++i) {
-            for (int j = 0; j < nn; ++j) {
-                double u = 0.0;
-                for (int sp = 0; sp < 2; ++sp) {
-                    const double *B = sp ? BB : BA;
-                    const double ui = B[0] - B[(i + 1) * n1] - B[j + 1] + B[(i + 1) * n1 + (j + 1)];
-                    u = sp ? u + ui : ui;
-                }
-                U[i * nn + j] = u;
-            }
-            V[i] = (BA[0] - BA[(i + 1) * n1]) + (BB[0] - BB[(i + 1) * n1]);
+    }
+    __syncthreads();
+    for (int idx = tid; idx < n1 * n1; idx += nt) {
+        BA[idx] = Bn[0][idx];
+        BB[idx] = Bn[1][idx];
+    }
+    // U = UA + UB, V = VA + VB (src/FieldAX.cc:352-381, src/Model_AB.cc:239-246)
+    for (int idx = tid; idx < nn * nn; idx += nt) {
+        const int i = idx / nn, j = idx % nn;
+        double u = 0.0;
+        for (int sp = 0; sp < 2; ++sp) {
+            const double *B = Bn[sp];
+            const double ui = B[0] - B[(i + 1) * n1] - B[j + 1] + B[(i + 1) * n1 + (j + 1)];
+            u = sp ? u + ui : ui;
         }
+        U[idx] = u;
+    }
+    for (int i = tid; i < nn; i += nt) V[i] = (Bn[0][0] - Bn[0][(i + 1) * n1]) + (Bn[1][0] - Bn[1][(i + 1) * n1]);
+    __syncthreads();
+    if (tid == 0) {
+        // LU with partial pivoting in LDS (arma::solve -> dgesv, src/FieldAX.cc:386-390)
         int ok = 1;
         for (int k = 0; k < nn && ok; ++k) {
             int p = k;
@@ -303,10 +324,10 @@ __global__ void k_anderson_solve(AndersonArgs a, const double *__restrict__ part
         }
         for (int i = nn - 1; i >= 0; --i) {
             double s = V[i];
-            for (int j = i + 1; j < nn; ++j) s -= U[i * nn + j] * C[j];
-            C[i] = ok ? s / U[i * nn + i] : 0.0;
+            for (int j = i + 1; j < nn; ++j) s -= U[i * nn + j] * V[j];
+            V[i] = ok ? s / U[i * nn + i] : 0.0;
         }
-        for (int i = 0; i < nn && !ok; ++i) C[i] = 0.0;
+        for (int i = 0; i < nn; ++i) C[i] = ok ? V[i] : 0.0;
         obs[OBS_RESIDUAL] = sqrt((sums[0] + sums[a.nn + 1]) / (sums[nv - 2] + sums[nv - 1]));
         obs[OBS_SINGULAR] = ok ? 0.0 : 1.0;
     }
@@ -391,7 +412,7 @@ hipError_t launch_anderson_d(const AndersonArgs &a, const double *phiA, const do
 
 hipError_t launch_anderson_solve(const AndersonArgs &a, const double *partials, double *BA, double *BB, double *C,
                                  double *obs, hipStream_t st) {
-    hipLaunchKernelGGL(k_anderson_solve, dim3(1), dim3(64), 0, st, a, partials, kRedBlocks, BA, BB, C, obs);
+    hipLaunchKernelGGL(k_anderson_solve, dim3(1), dim3(256), 0, st, a, partials, kRedBlocks, BA, BB, C, obs);
     return hipGetLastError();
 }
 

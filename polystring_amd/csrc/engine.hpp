@@ -98,8 +98,11 @@ template <typename T>
 hipError_t launch_grad_accum(const GradStep<T> *steps, int n, double ds, double *G, long M, hipStream_t st);
 int xdot_blocks(const Shape &sh);
 // complex elements of one stored spectrum of the fused adjoint, -1 when the
-// grid has no full-tile x-pencil kernel
-template <typename T> long xstore_elems(const Shape &sh);  // workgroups per descriptor (partials per descriptor = 3x this)
+// grid has no full-tile x-pencil kernel; the x-pencil variant actually run
+// for a requested one, and its workgroups per spectrum
+template <typename T> long xstore_elems(const Shape &sh, int xvariant);
+int xpass_variant(const Shape &sh, int xvariant);
+int xpass_blocks(const Shape &sh, int xvariant);  // workgroups per descriptor (partials per descriptor = 3x this)
 hipError_t launch_sum_rows(const double *partials, long nrows, int nval, double *scratch, double *out,
                            hipStream_t st);
 bool xpass_supported(int Lx);
@@ -110,7 +113,8 @@ hipError_t launch_plane(const Shape &sh, const Roots<T> &rt, const PlaneDesc<T> 
                         hipStream_t st, int variant = 0);
 template <typename T>
 hipError_t launch_xpass(const Shape &sh, const Roots<T> &rt, const XDesc<T> *d_desc, int nlanes,
-                        hipStream_t st);
+                        hipStream_t st,
+                        int xvariant = 0);
 
 // Elementwise / reduction launchers (kernels.hip).
 template <typename T>

@@ -597,10 +597,14 @@ k_xdot(const DotDesc<T> *__restrict__ desc, Roots<T> rt, int ncol, int hz, int n
 // its first stage and ends in registers, where K is applied; the inverse
 // starts from those registers and writes HBM in its last stage.  Only the
 // middle stages use LDS.  Needs ncol to be a multiple of C (full tiles).
-template <typename T, int LX> struct X3Cfg {
-    static constexpr int C = 16;
-    // LX*C/8 threads (one radix-8 butterfly each) rounded up to a power of two
-    static constexpr int NT0 = LX * C / 8;
+// W = 0: 16-column tiles; W = 1: 32-column tiles with the same threads, so
+// each wave reads twice as many adjacent columns per row (64 B FP32, 128 B
+// FP64 segments) and half as many workgroups are launched.
+template <typename T, int LX, int W = 0> struct X3Cfg {
+    static constexpr int C = W ? 32 : 16;
+    // LX*16/8 threads (one radix-8 butterfly each for a 16-column tile)
+    // rounded up to a power of two
+    static constexpr int NT0 = LX * 16 / 8;
     static constexpr int NT = NT0 <= 64 ? 64 : NT0 <= 128 ? 128 : NT0 <= 256 ? 256 : 512;
     static constexpr int NW = NT / 64;
     static constexpr int CPW = C / NW;
@@ -612,10 +616,10 @@ template <typename T, int LX> struct X3Cfg {
     static_assert(C % NW == 0, "tile columns must split evenly over waves");
 };
 
-template <typename T, int LX>
-__global__ void __launch_bounds__((X3Cfg<T, LX>::NT))
+template <typename T, int LX, int W>
+__global__ void __launch_bounds__((X3Cfg<T, LX, W>::NT))
 k_xpass3(const XDesc<T> *__restrict__ descs, Roots<T> rt, int ncol, int hz) {
-    using X = X3Cfg<T, LX>;
+    using X = X3Cfg<T, LX, W>;
     constexpr int CPW = X::CPW, RS = X::RS;
     using F = WaveEnds<LX, false, CPW>;
     using I = WaveEnds<LX, true, CPW>;
@@ -804,12 +808,15 @@ hipError_t launch_plane(const Shape &sh, const Roots<T> &rt, const PlaneDesc<T> 
 
 template <typename T>
 hipError_t launch_xpass(const Shape &sh, const Roots<T> &rt, const XDesc<T> *d_desc, int nlanes,
-                        hipStream_t st) {
+                        hipStream_t st, int xvariant) {
     const int ncol = sh.Ly * sh.Hz;
 #define PO_CASE(a)                                                                              \
     if (sh.Lx == a) {                                                                           \
-        if (ncol % X3Cfg<T, a>::C == 0) {                                                       \
-            hipLaunchKernelGGL((k_xpass3<T, a>), dim3(ncol / X3Cfg<T, a>::C, nlanes), dim3(X3Cfg<T, a>::NT), 0, st, \
+        if (xvariant == 1 && ncol % X3Cfg<T, a, 1>::C == 0) {                                   \
+            hipLaunchKernelGGL((k_xpass3<T, a, 1>), dim3(ncol / X3Cfg<T, a, 1>::C, nlanes),      \
+                               dim3(X3Cfg<T, a, 1>::NT), 0, st, d_desc, rt, ncol, sh.Hz);       \
+        } else if (ncol % X3Cfg<T, a>::C == 0) {                                                \
+            hipLaunchKernelGGL((k_xpass3<T, a, 0>), dim3(ncol / X3Cfg<T, a>::C, nlanes), dim3(X3Cfg<T, a>::NT), 0, st, \
                                d_desc, rt, ncol, sh.Hz);                                        \
         } else {                                                                                \
             const int nt = (ncol + XCfg<T, a>::C - 1) / XCfg<T, a>::C;                          \
@@ -823,11 +830,23 @@ hipError_t launch_xpass(const Shape &sh, const Roots<T> &rt, const XDesc<T> *d_d
     return hipErrorInvalidValue;
 }
 
-template <typename T> long xstore_elems(const Shape &sh) {
+// the x-pencil variant launch_xpass runs for a requested one
+int xpass_variant(const Shape &sh, int xvariant) {
     const int ncol = sh.Ly * sh.Hz;
+    return (xvariant == 1 && ncol % 32 == 0) ? 1 : 0;
+}
+
+template <typename T> long xstore_elems(const Shape &sh, int xvariant) {
+    const int ncol = sh.Ly * sh.Hz;
+    const int v = xpass_variant(sh, xvariant);
 #define PO_CASE(a)                                                                                  \
     if (sh.Lx == a) {                                                                               \
-        using X = X3Cfg<T, a>;                                                                      \
+        if (v == 1) {                                                                               \
+            using X = X3Cfg<T, a, 1>;                                                               \
+            using F = WaveEnds<a, false, X::CPW>;                                                   \
+            return (long)(ncol / X::C) * X::NW * F::ITL * F::RL * 64;                               \
+        }                                                                                           \
+        using X = X3Cfg<T, a, 0>;                                                                   \
         using F = WaveEnds<a, false, X::CPW>;                                                       \
         return ncol % X::C ? -1L : (long)(ncol / X::C) * X::NW * F::ITL * F::RL * 64;               \
     }
@@ -835,13 +854,16 @@ template <typename T> long xstore_elems(const Shape &sh) {
 #undef PO_CASE
     return -1;
 }
-template long xstore_elems<double>(const Shape &);
-template long xstore_elems<float>(const Shape &);
+template long xstore_elems<double>(const Shape &, int);
+template long xstore_elems<float>(const Shape &, int);
 
-int xdot_blocks(const Shape &sh) {
+int xpass_blocks(const Shape &sh, int xvariant) {
     const int ncol = sh.Ly * sh.Hz;
+    if (xpass_variant(sh, xvariant) == 1) return ncol / 32;
     return ncol % 16 == 0 ? ncol / 16 : (ncol + 15) / 16;
 }
+
+int xdot_blocks(const Shape &sh) { return xpass_blocks(sh, 0); }
 
 template <typename T>
 hipError_t launch_xdot(const Shape &sh, const Roots<T> &rt, const DotDesc<T> *d_desc, int ndesc, hipStream_t st) {
@@ -873,8 +895,8 @@ template hipError_t launch_plane<double>(const Shape &, const Roots<double> &, c
 template hipError_t launch_plane<float>(const Shape &, const Roots<float> &, const PlaneDesc<float> *, int,
                                         hipStream_t, int);
 template hipError_t launch_xpass<double>(const Shape &, const Roots<double> &, const XDesc<double> *, int,
-                                         hipStream_t);
+                                         hipStream_t, int);
 template hipError_t launch_xpass<float>(const Shape &, const Roots<float> &, const XDesc<float> *, int,
-                                        hipStream_t);
+                                        hipStream_t, int);
 
 }  // namespace po

@@ -430,33 +430,48 @@ template <typename T> struct Program {
     int run(const Shape &sh, const Roots<T> &rt, hipStream_t st) const {
         for (const auto &l : seq) {
             hipError_t e = l.plane ? launch_plane<T>(sh, rt, pd.p + l.off, l.n, st, plane_variant)
-                                   : launch_xpass<T>(sh, rt, xd.p + l.off, l.n, st);
+                                   : launch_xpass<T>(sh, rt, xd.p + l.off, l.n, st, x_variant);
             if (e != hipSuccess) return fail(PO_ERR_DEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
         }
         return PO_OK;
     }
 
-    // Autotuner: times the whole program once per plane-kernel variant (the
-    // workgroup size of the yz-plane kernel) on the live buffers and keeps
+    // Autotuner: times the whole program once per launch-shape combination
+    // -- the yz-plane kernel's workgroup size (512 or 1024 threads) and the
+    // x-pencil tile width (16 or 32 columns) -- on the live buffers and keeps
     // the fastest.  Every slice and spectrum the program writes is rewritten
-    // by the next real run, so tuning leaves no trace in the results.
+    // by the next real run, so tuning leaves no trace in the results.  All
+    // variants compute the same arithmetic in the same order.
     int tune(const Shape &sh, const Roots<T> &rt, hipStream_t st) {
-        const int nvar = (long)sh.Ly * sh.Lz >= 8192 ? 2 : 1;
-        if (nvar == 1) return PO_OK;
-        // PO_PLANE_VARIANT=0|1 pins the choice (reproducible launch shapes)
+        const int npv = (long)sh.Ly * sh.Lz >= 8192 ? 2 : 1;
+        const int nxv = xpass_variant(sh, 1) == 1 ? 2 : 1;
+        // PO_PLANE_VARIANT / PO_XPASS_VARIANT = 0|1 pin the choices (reproducible launch shapes)
+        int pin_p = -1, pin_x = -1;
         if (const char *env = std::getenv("PO_PLANE_VARIANT")) {
-            const int v = std::atoi(env);
-            if (v < 0 || v >= nvar) return fail(PO_ERR_INVALID, "PO_PLANE_VARIANT out of range");
-            plane_variant = v;
+            pin_p = std::atoi(env);
+            if (pin_p < 0 || pin_p >= npv) return fail(PO_ERR_INVALID, "PO_PLANE_VARIANT out of range");
+        }
+        if (const char *env = std::getenv("PO_XPASS_VARIANT")) {
+            pin_x = std::atoi(env);
+            if (pin_x < 0 || pin_x >= nxv) return fail(PO_ERR_INVALID, "PO_XPASS_VARIANT out of range");
+        }
+        std::vector<std::pair<int, int>> cand;
+        for (int p = 0; p < npv; ++p)
+            for (int x = 0; x < nxv; ++x)
+                if ((pin_p < 0 || p == pin_p) && (pin_x < 0 || x == pin_x)) cand.push_back({p, x});
+        if (cand.size() == 1) {
+            plane_variant = cand[0].first;
+            x_variant = cand[0].second;
             return PO_OK;
         }
         hipEvent_t a, b;
         PO_HIP(hipEventCreate(&a));
         PO_HIP(hipEventCreate(&b));
         float best = 1e30f;
-        int pick = 0;
-        for (int v = 0; v < nvar; ++v) {
-            plane_variant = v;
+        std::pair<int, int> pick = cand[0];
+        for (const auto &c : cand) {
+            plane_variant = c.first;
+            x_variant = c.second;
             int rc = run(sh, rt, st);  // warm-up
             if (rc) return rc;
             PO_HIP(hipEventRecord(a, st));
@@ -466,12 +481,13 @@ template <typename T> struct Program {
             PO_HIP(hipEventSynchronize(b));
             float ms = 0.f;
             PO_HIP(hipEventElapsedTime(&ms, a, b));
-            tuned_ms[v] = ms / 2;
-            if (ms < best) { best = ms; pick = v; }
+            tuned_ms[c.first * 2 + c.second] = ms / 2;
+            if (ms < best) { best = ms; pick = c; }
         }
         (void)hipEventDestroy(a);
         (void)hipEventDestroy(b);
-        plane_variant = pick;
+        plane_variant = pick.first;
+        x_variant = pick.second;
         return PO_OK;
     }
 
@@ -507,7 +523,8 @@ template <typename T> struct Program {
 
     long slice_elems = 0;  // M, set before build()
     int plane_variant = 0;
-    float tuned_ms[2] = {0.f, 0.f};
+    int x_variant = 0;
+    float tuned_ms[4] = {0.f, 0.f, 0.f, 0.f};  // [plane variant * 2 + x variant]
     int use_graph = -1;
     hipGraphExec_t exec = nullptr;
 };

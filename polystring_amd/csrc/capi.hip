@@ -288,7 +288,8 @@ template <typename T> struct Model : po_model {
             int rc = (fused_on ? prog_adj : prog).run_graph(sh, rt, st);
             if (rc) return rc;
             if (fused_on) {
-                PO_HIP(launch_sum_rows(fusedPart.p, (long)prog_adj.ndot * xpass_blocks(sh, prog_adj.x_variant), 3,
+                PO_HIP(launch_sum_rows(fusedPart.p,
+                                       (long)prog_adj.ndot * xpass_blocks(sh, prog_adj.x_variant, prog_adj.split), 3,
                                        part.p, fusedSum.p, st));
                 fused_valid = true;
             } else {
@@ -367,7 +368,7 @@ template <typename T> struct Model : po_model {
             return PO_OK;
         }
         const bool rqm4 = desc.algo == PO_RQM4;
-        const long elems = xstore_elems<T>(sh, prog.x_variant);
+        const long elems = xstore_elems<T>(sh, prog.x_variant, prog.split);
         const int N = (sA - 1) + (sB - 1);
         if (elems < 0 || N % 2 != 0 || (!rqm4 && !cell.ortho))
             return fail(PO_ERR_INVALID, "the fused adjoint needs Ly*(Lz/2+1) divisible by 16, an even number of "
@@ -376,7 +377,7 @@ template <typename T> struct Model : po_model {
         const int nk = rqm4 ? 3 : 1;
         AdjointArena<T> a;
         a.elems = elems;
-        a.nblk = xpass_blocks(sh, prog.x_variant);
+        a.nblk = xpass_blocks(sh, prog.x_variant, prog.split);
         PO_HIP(fusedSpec.alloc((size_t)(N / 2) * 2 * nk * elems));
         PO_HIP(fusedPart.alloc((size_t)(N / 2) * 2 * nk * a.nblk * 3));
         PO_HIP(fusedSum.alloc(3));
@@ -390,6 +391,7 @@ template <typename T> struct Model : po_model {
         }
         prog_adj.plane_variant = prog.plane_variant;
         prog_adj.x_variant = prog.x_variant;
+        prog_adj.split = prog.split;
         fused_on = true;
         return PO_OK;
     }

@@ -81,6 +81,7 @@ struct Shape {
 // Device tables of N-th roots of unity exp(-2 pi i m / N), m in [0, N).
 template <typename T> struct Roots {
     const cx<T> *x, *y, *zh, *z;  // lengths Lx, Ly, Lz/2, Lz
+    const cx<T> *yh;              // length Ly/2 (split-y plane kernels)
 };
 
 bool plane_supported(int Ly, int Lz);
@@ -100,21 +101,23 @@ int xdot_blocks(const Shape &sh);
 // complex elements of one stored spectrum of the fused adjoint, -1 when the
 // grid has no full-tile x-pencil kernel; the x-pencil variant actually run
 // for a requested one, and its workgroups per spectrum
-template <typename T> long xstore_elems(const Shape &sh, int xvariant);
-int xpass_variant(const Shape &sh, int xvariant);
-int xpass_blocks(const Shape &sh, int xvariant);  // workgroups per descriptor (partials per descriptor = 3x this)
+template <typename T> long xstore_elems(const Shape &sh, int xvariant, int split);
+int xpass_variant(const Shape &sh, int xvariant, int split);
+int xpass_blocks(const Shape &sh, int xvariant, int split);  // workgroups per descriptor (partials per descriptor = 3x this)
 hipError_t launch_sum_rows(const double *partials, long nrows, int nval, double *scratch, double *out,
                            hipStream_t st);
 bool xpass_supported(int Lx);
 
 // variant 0: default workgroup size; 1: 1024 threads (planes of >= 8192 reals)
+// split = 1: the split-y layout (half-plane kernels + radix-2 y stage in the
+// x-pencil kernel); plane and x-pencil launches of one program must agree.
 template <typename T>
 hipError_t launch_plane(const Shape &sh, const Roots<T> &rt, const PlaneDesc<T> *d_desc, int nlanes,
-                        hipStream_t st, int variant = 0);
+                        hipStream_t st, int variant = 0, int split = 0);
 template <typename T>
 hipError_t launch_xpass(const Shape &sh, const Roots<T> &rt, const XDesc<T> *d_desc, int nlanes,
-                        hipStream_t st,
-                        int xvariant = 0);
+                        hipStream_t st, int xvariant = 0, int split = 0);
+bool split_supported(const Shape &sh);
 
 // Elementwise / reduction launchers (kernels.hip).
 template <typename T>

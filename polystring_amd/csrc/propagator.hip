@@ -289,24 +289,35 @@ __device__ __forceinline__ void rows_c2r_split(cx<T> *rows, const cx<T> *__restr
 // inverse z transform produces and the first stage of the forward one takes.
 // The y transforms read their input spectrum from and write their output
 // spectrum to HBM directly in their first / last stage.
-template <typename T, int LY, int LZ, int NTV>
-__global__ void __launch_bounds__((PlaneCfg<T, LY, LZ, NTV>::NT))
+//
+// SPLIT = 1 is the split-y form: a workgroup owns the LY/2 rows y = 2j + h
+// (h = blockIdx.z) of its plane, and the y transform it runs is the
+// LY/2-point transform of those rows.  Its spectrum is the half h of the
+// plane's spectrum in the split layout [h][k'][kz] (k' < LY/2): E (even
+// rows) or O (odd rows).  The radix-2 y stage that joins them,
+// Y[k'] = E + W^k' O, Y[k'+LY/2] = E - W^k' O, runs in the x-pencil kernel
+// k_xpass4, where both halves of a column meet.  Half planes take half the
+// LDS, so two workgroups share a CU and overlap their memory and LDS phases.
+template <typename T, int LY, int LZ, int NTV, int SPLIT = 0>
+__global__ void __launch_bounds__((PlaneCfg<T, (SPLIT ? LY / 2 : LY), LZ, NTV>::NT))
 k_plane3(const PlaneDesc<T> *__restrict__ descs, Roots<T> rt, T invM) {
-    using C = PlaneCfg<T, LY, LZ, NTV>;
+    constexpr int LYW = SPLIT ? LY / 2 : LY;  // rows (and y transform length) of this workgroup
+    using C = PlaneCfg<T, LYW, LZ, NTV>;
     constexpr int H = C::H, HZ = C::HZ, RS = C::RS, NW = C::NW, RPW = C::RPW, CPW = C::CPW;
-    constexpr int NE = C::NE, NC = C::NC;
     using ZF = WaveEnds<H, false, RPW>;
     using ZI = WaveEnds<H, true, RPW>;
     static_assert(ZI::RL == ZF::RF && ZI::ITL == ZF::ITF, "z transform ends must match");
     constexpr int RZ = ZF::RF, ITZ = ZF::ITF, NBZ = H / RZ, TOTZ = RPW * NBZ;
-    using YI = WaveEnds<LY, true, CPW>;
-    using YF = WaveEnds<LY, false, CPW>;
-    using YI1 = WaveEnds<LY, true, 1>;
-    using YF1 = WaveEnds<LY, false, 1>;
+    using YI = WaveEnds<LYW, true, CPW>;
+    using YF = WaveEnds<LYW, false, CPW>;
+    using YI1 = WaveEnds<LYW, true, 1>;
+    using YF1 = WaveEnds<LYW, false, 1>;
     __shared__ cx<T> buf[C::NL];
     const PlaneDesc<T> d = descs[blockIdx.y];
-    const long ro = (long)blockIdx.x * NE;
-    const long co = (long)blockIdx.x * NC;
+    const int hh = SPLIT ? (int)blockIdx.z : 0;
+    const long ro = (long)blockIdx.x * (LY * LZ);
+    const long co = (long)blockIdx.x * (LY * HZ) + (long)hh * (LYW * HZ);
+    const cx<T> *ry = SPLIT ? rt.yh : rt.y;  // roots of order LYW
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     cx<T> *rows = buf + w * RPW * RS;
 
@@ -314,7 +325,8 @@ k_plane3(const PlaneDesc<T> *__restrict__ descs, Roots<T> rt, T invM) {
     // a block-uniform base plus a 32-bit offset, so addresses stay one VGPR
     auto pair = [&](const T *base, int it, int r) -> const cx<T> * {
         const int b = lane + it * 64;
-        const unsigned off = (unsigned)((w * RPW + b / NBZ) * (LZ / 2) + b % NBZ + r * NBZ);
+        const int row = w * RPW + b / NBZ;
+        const unsigned off = (unsigned)((SPLIT ? 2 * row + hh : row) * (LZ / 2) + b % NBZ + r * NBZ);
         return reinterpret_cast<const cx<T> *>(base + ro) + off;
     };
     auto valid = [&](int it) { return (TOTZ % 64 == 0) || (lane + it * 64) < TOTZ; };
@@ -323,12 +335,12 @@ k_plane3(const PlaneDesc<T> *__restrict__ descs, Roots<T> rt, T invM) {
         __syncthreads();  // LDS free
         {
             cx<T> a[YI::ITF][YI::RF], b[YI::ITL][YI::RL];
-            WaveFFT<T, LY, true, CPW, true, IO_GLB, IO_LDS>::run(buf + w * CPW, 1, RS, rt.y, lane, src + co + w * CPW,
+            WaveFFT<T, LYW, true, CPW, true, IO_GLB, IO_LDS>::run(buf + w * CPW, 1, RS, ry, lane, src + co + w * CPW,
                                                                  1, HZ, nullptr, a, b);
         }
         if (w == NW - 1) {
             cx<T> a[YI1::ITF][YI1::RF], b[YI1::ITL][YI1::RL];
-            WaveFFT<T, LY, true, 1, true, IO_GLB, IO_LDS>::run(buf + H, 1, RS, rt.y, lane, src + co + H, 1, HZ,
+            WaveFFT<T, LYW, true, 1, true, IO_GLB, IO_LDS>::run(buf + H, 1, RS, ry, lane, src + co + H, 1, HZ,
                                                                nullptr, a, b);
         }
         __syncthreads();
@@ -351,12 +363,12 @@ k_plane3(const PlaneDesc<T> *__restrict__ descs, Roots<T> rt, T invM) {
         __syncthreads();
         {
             cx<T> a[YF::ITF][YF::RF], b[YF::ITL][YF::RL];
-            WaveFFT<T, LY, false, CPW, true, IO_LDS, IO_GLB>::run(buf + w * CPW, 1, RS, rt.y, lane, nullptr, 1, HZ,
+            WaveFFT<T, LYW, false, CPW, true, IO_LDS, IO_GLB>::run(buf + w * CPW, 1, RS, ry, lane, nullptr, 1, HZ,
                                                                   dst + co + w * CPW, a, b);
         }
         if (w == NW - 1) {
             cx<T> a[YF1::ITF][YF1::RF], b[YF1::ITL][YF1::RL];
-            WaveFFT<T, LY, false, 1, true, IO_LDS, IO_GLB>::run(buf + H, 1, RS, rt.y, lane, nullptr, 1, HZ,
+            WaveFFT<T, LYW, false, 1, true, IO_LDS, IO_GLB>::run(buf + H, 1, RS, ry, lane, nullptr, 1, HZ,
                                                                 dst + co + H, a, b);
         }
     };
@@ -696,6 +708,107 @@ k_xpass3(const XDesc<T> *__restrict__ descs, Roots<T> rt, int ncol, int hz) {
     }
 }
 
+// x-pencil kernel of the split-y layout (see k_plane3 with SPLIT = 1).  A
+// wave owns CPW adjacent half-columns c = k' Hz + kz and takes both halves of
+// each: E at c and O at Lyh Hz + c of every x plane.  Forward x transforms
+// of both, then in registers the radix-2 y stage Y_lo = E + t O, Y_hi = E - t O
+// with t = W_Ly^k' (ky = k' and k' + Ly/2), K and the fused adjoint on the
+// full spectrum, the inverse stage E' = Y_lo + Y_hi, O' = (Y_lo - Y_hi) t*,
+// and inverse x transforms.  Needs Ly Hz / 2 to be a multiple of C.
+template <typename T, int LX, int W>
+__global__ void __launch_bounds__((X3Cfg<T, LX, W>::NT))
+k_xpass4(const XDesc<T> *__restrict__ descs, Roots<T> rt, int ncol, int hz, int lyh) {
+    using X = X3Cfg<T, LX, W>;
+    constexpr int CPW = X::CPW, RS = X::RS;
+    using F = WaveEnds<LX, false, CPW>;
+    using I = WaveEnds<LX, true, CPW>;
+    static_assert(F::RL == I::RF && F::ITL == I::ITF, "x transform ends must match");
+    constexpr int RX = F::RL, ITX = F::ITL, NBX = LX / RX, TOTX = CPW * NBX;
+    __shared__ cx<T> buf[LX * RS];
+    const XDesc<T> d = descs[blockIdx.y];
+    const int half = ncol >> 1;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int cw = blockIdx.x * X::C + w * CPW;  // first half-column of this wave
+    cx<T> *lds = buf + w * CPW;
+    cx<T> ve[ITX][RX], vo[ITX][RX];
+    {
+        cx<T> a[F::ITF][F::RF];
+        WaveFFT<T, LX, false, CPW, true, IO_GLB, IO_REG>::run(lds, 1, RS, rt.x, lane, d.S + cw, 1, ncol, nullptr, a,
+                                                              ve);
+    }
+    {
+        cx<T> a[F::ITF][F::RF];
+        WaveFFT<T, LX, false, CPW, true, IO_GLB, IO_REG>::run(lds, 1, RS, rt.x, lane, d.S + half + cw, 1, ncol,
+                                                              nullptr, a, vo);
+    }
+    // stored spectra: the lo set (ky = k') then the hi set (ky = k' + Ly/2) of this wave
+    const unsigned sbase = (blockIdx.x * X::NW + w) * (unsigned)(2 * ITX * RX * 64);
+    double acc[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int it = 0; it < ITX; ++it) {
+        const int b = lane + it * 64;
+        if ((TOTX % 64 == 0) || b < TOTX) {
+            const int c = cw + b % CPW, j = b / CPW;
+            const int kp = c / hz, kz = c % hz;
+            const cx<T> t = gload(rt.y + kp);
+            const T fy0 = gload(d.fy + kp), fy1 = gload(d.fy + kp + lyh), fz = gload(d.fz + kz);
+            const double wz = (kz == 0 || kz == hz - 1) ? 1.0 : 2.0;
+#pragma unroll
+            for (int r = 0; r < RX; ++r) {
+                const int x = j + r * NBX;
+                const cx<T> to = cmul<false, T>(vo[it][r], t);
+                cx<T> lo = add<T>(ve[it][r], to), hi = sub<T>(ve[it][r], to);
+                const unsigned so = sbase + (unsigned)((it * RX + r) * 64 + lane);
+                if (d.Sstore) {
+                    gstore(d.Sstore + so, lo);
+                    gstore(d.Sstore + so + ITX * RX * 64, hi);
+                }
+                const T fx = gload(d.fx + x);
+                const T f0 = fx * fy0 * fz, f1 = fx * fy1 * fz;
+                if (d.Sdot) {
+                    const cx<T> y0 = gload(d.Sdot + so), y1 = gload(d.Sdot + so + ITX * RX * 64);
+                    const double p0 = wz * (double)f0 * ((double)y0.x * (double)lo.x + (double)y0.y * (double)lo.y);
+                    const double p1 = wz * (double)f1 * ((double)y1.x * (double)hi.x + (double)y1.y * (double)hi.y);
+                    const double gx = gload(d.gx + x);
+                    acc[0] += (p0 + p1) * gx;
+                    acc[1] += p0 * (double)gload(d.gy + kp) + p1 * (double)gload(d.gy + kp + lyh);
+                    acc[2] += (p0 + p1) * (double)gload(d.gz + kz);
+                }
+                lo = scl<T>(lo, f0);
+                hi = scl<T>(hi, f1);
+                ve[it][r] = add<T>(lo, hi);
+                vo[it][r] = cmul<true, T>(sub<T>(lo, hi), t);
+            }
+        }
+    }
+    {
+        cx<T> bb[I::ITL][I::RL];
+        WaveFFT<T, LX, true, CPW, true, IO_REG, IO_GLB>::run(lds, 1, RS, rt.x, lane, nullptr, 1, ncol,
+                                                             (d.Sout ? d.Sout : d.S) + cw, ve, bb);
+    }
+    {
+        cx<T> bb[I::ITL][I::RL];
+        WaveFFT<T, LX, true, CPW, true, IO_REG, IO_GLB>::run(lds, 1, RS, rt.x, lane, nullptr, 1, ncol,
+                                                             (d.Sout ? d.Sout : d.S) + half + cw, vo, bb);
+    }
+    if (d.Sdot) {
+        double *red = reinterpret_cast<double *>(buf);
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            for (int off = 32; off > 0; off >>= 1) acc[k] += __shfl_down(acc[k], off, 64);
+        __syncthreads();
+        if (lane == 0)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) red[k * X::NW + w] = acc[k];
+        __syncthreads();
+        if (threadIdx.x < 3) {
+            double sum = 0.0;
+            for (int i = 0; i < X::NW; ++i) sum += red[threadIdx.x * X::NW + i];
+            d.partials[blockIdx.x * 3 + threadIdx.x] = d.coef * sum;
+        }
+    }
+}
+
 // Stress dot kernel (see DotDesc): x transforms of both spectra of a pair in
 // registers, then the k-space weighted products, reduced per workgroup.
 template <typename T, int LX>
@@ -779,12 +892,33 @@ bool xpass_supported(int Lx) {
     return false;
 }
 
+// grids that have the split-y plane and x-pencil kernels
+bool split_supported(const Shape &sh) {
+    const int half = sh.Ly * sh.Hz / 2;
+    return sh.Lz > 1 && sh.Ly >= 32 && sh.Ly % 2 == 0 && half % 16 == 0 && plane_supported(sh.Ly, sh.Lz) &&
+           xpass_supported(sh.Lx);
+}
+
 template <typename T>
 hipError_t launch_plane(const Shape &sh, const Roots<T> &rt, const PlaneDesc<T> *d_desc, int nlanes,
-                        hipStream_t st, int variant) {
+                        hipStream_t st, int variant, int split) {
     const T invM = (T)(1.0 / (double)sh.M);
 #define PO_CASE(a, b)                                                                                 \
     if (sh.Ly == a && sh.Lz == b) {                                                                   \
+        if constexpr (a >= 32 && b > 1) {                                                             \
+            if (split) {                                                                              \
+                if constexpr ((a / 2) * b >= 8192) {                                                  \
+                    if (variant == 1) {                                                               \
+                        hipLaunchKernelGGL((k_plane3<T, a, b, 1024, 1>), dim3(sh.Lx, nlanes, 2), dim3(1024), 0, st, \
+                                           d_desc, rt, invM);                                         \
+                        return hipGetLastError();                                                     \
+                    }                                                                                 \
+                }                                                                                     \
+                hipLaunchKernelGGL((k_plane3<T, a, b, 0, 1>), dim3(sh.Lx, nlanes, 2),                 \
+                                   dim3(PlaneCfg<T, a / 2, b, 0>::NT), 0, st, d_desc, rt, invM);      \
+                return hipGetLastError();                                                             \
+            }                                                                                         \
+        }                                                                                             \
         if constexpr (a * b >= 8192) {                                                                \
             if (variant == 1) {                                                                       \
                 hipLaunchKernelGGL((k_plane3<T, a, b, 1024>), dim3(sh.Lx, nlanes), dim3(1024), 0, st, d_desc, rt, \
@@ -808,10 +942,20 @@ hipError_t launch_plane(const Shape &sh, const Roots<T> &rt, const PlaneDesc<T> 
 
 template <typename T>
 hipError_t launch_xpass(const Shape &sh, const Roots<T> &rt, const XDesc<T> *d_desc, int nlanes,
-                        hipStream_t st, int xvariant) {
+                        hipStream_t st, int xvariant, int split) {
     const int ncol = sh.Ly * sh.Hz;
+    const int half = ncol / 2;
 #define PO_CASE(a)                                                                              \
     if (sh.Lx == a) {                                                                           \
+        if (split) {                                                                            \
+            if (xvariant == 1 && half % X3Cfg<T, a, 1>::C == 0)                                 \
+                hipLaunchKernelGGL((k_xpass4<T, a, 1>), dim3(half / X3Cfg<T, a, 1>::C, nlanes),  \
+                                   dim3(X3Cfg<T, a, 1>::NT), 0, st, d_desc, rt, ncol, sh.Hz, sh.Ly / 2); \
+            else                                                                                \
+                hipLaunchKernelGGL((k_xpass4<T, a, 0>), dim3(half / X3Cfg<T, a, 0>::C, nlanes),  \
+                                   dim3(X3Cfg<T, a, 0>::NT), 0, st, d_desc, rt, ncol, sh.Hz, sh.Ly / 2); \
+            return hipGetLastError();                                                           \
+        }                                                                                       \
         if (xvariant == 1 && ncol % X3Cfg<T, a, 1>::C == 0) {                                   \
             hipLaunchKernelGGL((k_xpass3<T, a, 1>), dim3(ncol / X3Cfg<T, a, 1>::C, nlanes),      \
                                dim3(X3Cfg<T, a, 1>::NT), 0, st, d_desc, rt, ncol, sh.Hz);       \
@@ -831,16 +975,27 @@ hipError_t launch_xpass(const Shape &sh, const Roots<T> &rt, const XDesc<T> *d_d
 }
 
 // the x-pencil variant launch_xpass runs for a requested one
-int xpass_variant(const Shape &sh, int xvariant) {
-    const int ncol = sh.Ly * sh.Hz;
-    return (xvariant == 1 && ncol % 32 == 0) ? 1 : 0;
+int xpass_variant(const Shape &sh, int xvariant, int split) {
+    const int cols = split ? sh.Ly * sh.Hz / 2 : sh.Ly * sh.Hz;
+    return (xvariant == 1 && cols % 32 == 0) ? 1 : 0;
 }
 
-template <typename T> long xstore_elems(const Shape &sh, int xvariant) {
+template <typename T> long xstore_elems(const Shape &sh, int xvariant, int split) {
     const int ncol = sh.Ly * sh.Hz;
-    const int v = xpass_variant(sh, xvariant);
+    const int v = xpass_variant(sh, xvariant, split);
 #define PO_CASE(a)                                                                                  \
     if (sh.Lx == a) {                                                                               \
+        if (split) {                                                                                \
+            const int half = ncol / 2;                                                              \
+            if (v == 1) {                                                                           \
+                using X = X3Cfg<T, a, 1>;                                                           \
+                using F = WaveEnds<a, false, X::CPW>;                                               \
+                return (long)(half / X::C) * X::NW * 2 * F::ITL * F::RL * 64;                       \
+            }                                                                                       \
+            using X = X3Cfg<T, a, 0>;                                                               \
+            using F = WaveEnds<a, false, X::CPW>;                                                   \
+            return half % X::C ? -1L : (long)(half / X::C) * X::NW * 2 * F::ITL * F::RL * 64;       \
+        }                                                                                           \
         if (v == 1) {                                                                               \
             using X = X3Cfg<T, a, 1>;                                                               \
             using F = WaveEnds<a, false, X::CPW>;                                                   \
@@ -854,16 +1009,16 @@ template <typename T> long xstore_elems(const Shape &sh, int xvariant) {
 #undef PO_CASE
     return -1;
 }
-template long xstore_elems<double>(const Shape &, int);
-template long xstore_elems<float>(const Shape &, int);
+template long xstore_elems<double>(const Shape &, int, int);
+template long xstore_elems<float>(const Shape &, int, int);
 
-int xpass_blocks(const Shape &sh, int xvariant) {
-    const int ncol = sh.Ly * sh.Hz;
-    if (xpass_variant(sh, xvariant) == 1) return ncol / 32;
-    return ncol % 16 == 0 ? ncol / 16 : (ncol + 15) / 16;
+int xpass_blocks(const Shape &sh, int xvariant, int split) {
+    const int cols = split ? sh.Ly * sh.Hz / 2 : sh.Ly * sh.Hz;
+    if (xpass_variant(sh, xvariant, split) == 1) return cols / 32;
+    return cols % 16 == 0 ? cols / 16 : (cols + 15) / 16;
 }
 
-int xdot_blocks(const Shape &sh) { return xpass_blocks(sh, 0); }
+int xdot_blocks(const Shape &sh) { return xpass_blocks(sh, 0, 0); }
 
 template <typename T>
 hipError_t launch_xdot(const Shape &sh, const Roots<T> &rt, const DotDesc<T> *d_desc, int ndesc, hipStream_t st) {
@@ -891,12 +1046,12 @@ template hipError_t launch_xdot<float>(const Shape &, const Roots<float> &, cons
                                        hipStream_t);
 
 template hipError_t launch_plane<double>(const Shape &, const Roots<double> &, const PlaneDesc<double> *, int,
-                                         hipStream_t, int);
+                                         hipStream_t, int, int);
 template hipError_t launch_plane<float>(const Shape &, const Roots<float> &, const PlaneDesc<float> *, int,
-                                        hipStream_t, int);
+                                        hipStream_t, int, int);
 template hipError_t launch_xpass<double>(const Shape &, const Roots<double> &, const XDesc<double> *, int,
-                                         hipStream_t, int);
+                                         hipStream_t, int, int);
 template hipError_t launch_xpass<float>(const Shape &, const Roots<float> &, const XDesc<float> *, int,
-                                        hipStream_t, int);
+                                        hipStream_t, int, int);
 
 }  // namespace po

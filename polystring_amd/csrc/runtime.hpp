@@ -254,6 +254,7 @@ template <typename T> int make_roots(po_ctx *ctx, const Shape &sh, Roots<T> &rt)
     if ((rc = roots_for<T>(ctx, sh.Ly, &rt.y))) return rc;
     if ((rc = roots_for<T>(ctx, sh.Lz / 2, &rt.zh))) return rc;
     if ((rc = roots_for<T>(ctx, sh.Lz, &rt.z))) return rc;
+    if ((rc = roots_for<T>(ctx, sh.Ly / 2, &rt.yh))) return rc;
     return PO_OK;
 }
 
@@ -419,6 +420,8 @@ template <typename T> struct Program {
             }
             push_p(fin);
         }
+        has_tab = false;
+        for (const auto &x : X) has_tab = has_tab || x.tab != nullptr;
         PO_HIP(pd.alloc(P.size()));
         PO_HIP(xd.alloc(X.size() ? X.size() : 1));
         PO_HIP(hipMemcpyAsync(pd.p, P.data(), sizeof(PlaneDesc<T>) * P.size(), hipMemcpyHostToDevice, st));
@@ -429,8 +432,8 @@ template <typename T> struct Program {
 
     int run(const Shape &sh, const Roots<T> &rt, hipStream_t st) const {
         for (const auto &l : seq) {
-            hipError_t e = l.plane ? launch_plane<T>(sh, rt, pd.p + l.off, l.n, st, plane_variant)
-                                   : launch_xpass<T>(sh, rt, xd.p + l.off, l.n, st, x_variant);
+            hipError_t e = l.plane ? launch_plane<T>(sh, rt, pd.p + l.off, l.n, st, plane_variant, split)
+                                   : launch_xpass<T>(sh, rt, xd.p + l.off, l.n, st, x_variant, split);
             if (e != hipSuccess) return fail(PO_ERR_DEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
         }
         return PO_OK;
@@ -443,36 +446,50 @@ template <typename T> struct Program {
     // by the next real run, so tuning leaves no trace in the results.  All
     // variants compute the same arithmetic in the same order.
     int tune(const Shape &sh, const Roots<T> &rt, hipStream_t st) {
-        const int npv = (long)sh.Ly * sh.Lz >= 8192 ? 2 : 1;
-        const int nxv = xpass_variant(sh, 1) == 1 ? 2 : 1;
-        // PO_PLANE_VARIANT / PO_XPASS_VARIANT = 0|1 pin the choices (reproducible launch shapes)
-        int pin_p = -1, pin_x = -1;
-        if (const char *env = std::getenv("PO_PLANE_VARIANT")) {
-            pin_p = std::atoi(env);
-            if (pin_p < 0 || pin_p >= npv) return fail(PO_ERR_INVALID, "PO_PLANE_VARIANT out of range");
+        // PO_PLANE_VARIANT / PO_XPASS_VARIANT / PO_SPLIT = 0|1 pin the choices
+        // (reproducible launch shapes)
+        auto pin = [](const char *name, int n, int &out) -> int {
+            out = -1;
+            if (const char *env = std::getenv(name)) {
+                out = std::atoi(env);
+                if (out < 0 || out >= n) return fail(PO_ERR_INVALID, std::string(name) + " out of range");
+            }
+            return PO_OK;
+        };
+        const int nsp = (split_supported(sh) && !has_tab) ? 2 : 1;
+        int pin_p, pin_x, pin_s, rc;
+        // PO_SPLIT=1 means "split-y wherever the grid has the kernels"
+        if ((rc = pin("PO_SPLIT", 2, pin_s))) return rc;
+        if (pin_s >= nsp) pin_s = 0;
+        struct Cand { int p, x, s; };
+        std::vector<Cand> cand;
+        for (int sp = 0; sp < nsp; ++sp) {
+            const long rows = sp ? sh.Ly / 2 : sh.Ly;
+            const int npv = rows * sh.Lz >= 8192 ? 2 : 1;
+            const int nxv = xpass_variant(sh, 1, sp) == 1 ? 2 : 1;
+            if ((rc = pin("PO_PLANE_VARIANT", 2, pin_p)) || (rc = pin("PO_XPASS_VARIANT", 2, pin_x))) return rc;
+            for (int p = 0; p < npv; ++p)
+                for (int x = 0; x < nxv; ++x)
+                    if ((pin_p < 0 || p == pin_p) && (pin_x < 0 || x == pin_x) && (pin_s < 0 || sp == pin_s))
+                        cand.push_back({p, x, sp});
         }
-        if (const char *env = std::getenv("PO_XPASS_VARIANT")) {
-            pin_x = std::atoi(env);
-            if (pin_x < 0 || pin_x >= nxv) return fail(PO_ERR_INVALID, "PO_XPASS_VARIANT out of range");
-        }
-        std::vector<std::pair<int, int>> cand;
-        for (int p = 0; p < npv; ++p)
-            for (int x = 0; x < nxv; ++x)
-                if ((pin_p < 0 || p == pin_p) && (pin_x < 0 || x == pin_x)) cand.push_back({p, x});
+        if (cand.empty()) return fail(PO_ERR_INVALID, "pinned launch variant not available for this grid");
         if (cand.size() == 1) {
-            plane_variant = cand[0].first;
-            x_variant = cand[0].second;
+            plane_variant = cand[0].p;
+            x_variant = cand[0].x;
+            split = cand[0].s;
             return PO_OK;
         }
         hipEvent_t a, b;
         PO_HIP(hipEventCreate(&a));
         PO_HIP(hipEventCreate(&b));
         float best = 1e30f;
-        std::pair<int, int> pick = cand[0];
+        Cand pick = cand[0];
         for (const auto &c : cand) {
-            plane_variant = c.first;
-            x_variant = c.second;
-            int rc = run(sh, rt, st);  // warm-up
+            plane_variant = c.p;
+            x_variant = c.x;
+            split = c.s;
+            rc = run(sh, rt, st);  // warm-up
             if (rc) return rc;
             PO_HIP(hipEventRecord(a, st));
             for (int rep = 0; rep < 2 && rc == PO_OK; ++rep) rc = run(sh, rt, st);
@@ -481,13 +498,14 @@ template <typename T> struct Program {
             PO_HIP(hipEventSynchronize(b));
             float ms = 0.f;
             PO_HIP(hipEventElapsedTime(&ms, a, b));
-            tuned_ms[c.first * 2 + c.second] = ms / 2;
+            tuned_ms[c.s * 4 + c.p * 2 + c.x] = ms / 2;
             if (ms < best) { best = ms; pick = c; }
         }
         (void)hipEventDestroy(a);
         (void)hipEventDestroy(b);
-        plane_variant = pick.first;
-        x_variant = pick.second;
+        plane_variant = pick.p;
+        x_variant = pick.x;
+        split = pick.s;
         return PO_OK;
     }
 
@@ -524,7 +542,9 @@ template <typename T> struct Program {
     long slice_elems = 0;  // M, set before build()
     int plane_variant = 0;
     int x_variant = 0;
-    float tuned_ms[4] = {0.f, 0.f, 0.f, 0.f};  // [plane variant * 2 + x variant]
+    int split = 0;         // split-y layout of the spectra
+    bool has_tab = false;  // a full k table (general-cell OS2): no split-y kernel for it
+    float tuned_ms[8] = {};  // [split * 4 + plane variant * 2 + x variant]
     int use_graph = -1;
     hipGraphExec_t exec = nullptr;
 };

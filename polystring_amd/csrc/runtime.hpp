@@ -447,7 +447,7 @@ template <typename T> struct Program {
     // variants compute the same arithmetic in the same order.
     int tune(const Shape &sh, const Roots<T> &rt, hipStream_t st) {
         // PO_PLANE_VARIANT / PO_XPASS_VARIANT / PO_SPLIT = 0|1 pin the choices
-        // (reproducible launch shapes)
+        // (reproducible launch shapes) wherever the grid has that kernel
         auto pin = [](const char *name, int n, int &out) -> int {
             out = -1;
             if (const char *env = std::getenv(name)) {
@@ -473,7 +473,11 @@ template <typename T> struct Program {
                     if ((pin_p < 0 || p == pin_p) && (pin_x < 0 || x == pin_x) && (pin_s < 0 || sp == pin_s))
                         cand.push_back({p, x, sp});
         }
-        if (cand.empty()) return fail(PO_ERR_INVALID, "pinned launch variant not available for this grid");
+        if (cand.empty()) {
+            // a pin this grid has no kernel for falls back to the unpinned choice
+            for (int sp = 0; sp < nsp; ++sp)
+                if (pin_s < 0 || sp == pin_s) cand.push_back({0, 0, sp});
+        }
         if (cand.size() == 1) {
             plane_variant = cand[0].p;
             x_variant = cand[0].x;

@@ -223,10 +223,7 @@ def test_launch_variants_agree(ctx64, monkeypatch):
                 monkeypatch.setenv("PO_SPLIT", str(sp))
                 monkeypatch.setenv("PO_PLANE_VARIANT", str(pv))
                 monkeypatch.setenv("PO_XPASS_VARIANT", str(xv))
-                try:
-                    m = capi.ModelAB(ctx64, wA, wB, **kw)
-                except capi.POError:
-                    continue  # this combination has no kernel for the grid
+                m = capi.ModelAB(ctx64, wA, wB, **kw)  # pins without a kernel fall back
                 m.set_adjoint(True)
                 m.update(2)
                 out[(sp, pv, xv)] = (m.observe()["H"], m.get("wA"))

@@ -39,6 +39,8 @@ def _load():
     lib.or_klaplacian.argtypes = [_dp] + [ctypes.c_int] * 3 + [ctypes.c_double, _dp]
     for f in (lib.or_rqm4_solve, lib.or_os2_solve):
         f.argtypes = [ctypes.c_int] * 3 + [_dp, ctypes.c_double, ctypes.c_int, _dp, _dp]
+    lib.or_etdrk4_solve.argtypes = [ctypes.c_int] * 3 + [_dp, ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, _dp]
+    lib.or_etdrk4_coeffs.argtypes = [ctypes.c_long, _dp, ctypes.c_double, ctypes.c_int] + [_dp] * 6
     for f in (lib.or_simpson, lib.or_quad4_closed, lib.or_trapezoid):
         f.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_double, _dp]
         f.restype = ctypes.c_int
@@ -141,6 +143,22 @@ def os2_solve(laplace, ds, w, head, length):
     return qs
 
 
+def etdrk4_solve(k2neg, ds, w, head, length, npts=32):
+    shape = w.shape
+    qs = np.zeros((length,) + shape)
+    qs[0] = head
+    lib().or_etdrk4_solve(*shape, _p(np.ascontiguousarray(k2neg, dtype=np.float64)), ds, npts, length,
+                          _p(np.ascontiguousarray(w, dtype=np.float64)), _p(qs))
+    return qs
+
+
+def etdrk4_coeffs(k2neg, ds, npts=32):
+    k2neg = np.ascontiguousarray(k2neg, dtype=np.float64)
+    out = [np.zeros(k2neg.shape) for _ in range(6)]
+    lib().or_etdrk4_coeffs(k2neg.size, _p(k2neg), ds, npts, *[_p(o) for o in out])
+    return out  # E, E2, Q, f1, f2, f3
+
+
 def quadrature(kind, q, qc, cc, ds):
     """kind: 'simpson', 'quad4', 'trap'.  Returns None on an invalid slice count."""
     q = np.ascontiguousarray(q, dtype=np.float64)
@@ -173,7 +191,7 @@ class Model:
         code = CRYSTAL[cs] if isinstance(cs, str) else int(cs)
         self._h = lib().or_model_create(dim, *self.shape, code, a, b, c, al, be, ga,
                                         int(Ms[0]), int(Ms[1]), chiN, lam[0], lam[1],
-                                        n_anderson, 0 if mde.upper() == "RQM4" else 1,
+                                        n_anderson, {"RQM4": 0, "OS2": 1, "ETDRK4": 2}[mde.upper()],
                                         0 if contour.upper() == "SIMPSON" else 1,
                                         _p(wA0), _p(wB0))
         if not self._h:
@@ -227,7 +245,8 @@ class EMModel(Model):
         code = CRYSTAL[cs] if isinstance(cs, str) else int(cs)
         self._h = lib().or_em_create(dim, *self.shape, code, a, b, c, al, be, ga, int(Ms[0]), int(Ms[1]),
                                      chiN, lam[0], lam[1], lam[2], int(bool(mirror)), int(bool(compressible)),
-                                     0 if mde.upper() == "RQM4" else 1, 0 if contour.upper() == "SIMPSON" else 1,
+                                     {"RQM4": 0, "OS2": 1, "ETDRK4": 2}[mde.upper()],
+                                     0 if contour.upper() == "SIMPSON" else 1,
                                      _p(wA0), _p(wB0))
         if not self._h:
             raise ValueError("oracle model creation failed")

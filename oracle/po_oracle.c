@@ -12,6 +12,7 @@
  */
 #include "po_oracle.h"
 
+#include <complex.h>
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -387,6 +388,87 @@ void or_os2_solve(int Lx, int Ly, int Lz, const double *laplace, double ds,
     free(e); free(buf);
 }
 
+/* ETDRK4 (Cox-Matthews) coefficients for the linear operator L = -k^2
+ * (src/Etdrk4_PBC.cc:43-90): E = e^{ds L}, E2 = e^{ds L / 2}, and Q, f1, f2,
+ * f3 by the Kassam-Trefethen contour average over the npts points
+ * r_j = exp(i pi (j + 1/2) / npts) of the upper unit semicircle around ds L:
+ *   Q  = ds <Re (e^{z/2} - 1) / z>,
+ *   f1 = ds <Re (e^z (z^2 - 3z + 4) - z - 4) / z^3>,
+ *   f2 = ds <Re (e^z (z - 2) + z + 2) / z^3>,
+ *   f3 = ds <Re (e^z (4 - z) - z^2 - 3z - 4) / z^3>,   z = ds L + r_j. */
+void or_etdrk4_coeffs(long M, const double *k2neg, double ds, int npts, double *E, double *E2, double *Q,
+                      double *f1, double *f2, double *f3) {
+#pragma omp parallel for schedule(static) if (M > 4096)
+    for (long i = 0; i < M; i++) {
+        const double L = k2neg[i];
+        E[i] = exp(ds * L);
+        E2[i] = exp(ds * L / 2.0);
+        double sq = 0, s1 = 0, s2 = 0, s3 = 0;
+        for (int j = 0; j < npts; j++) {
+            const double ang = 3.14159265358979323846 * (j + 0.5) / npts;
+            const double complex z = ds * L + cexp(I * ang);
+            const double complex ez = cexp(z), z2 = z * z, z3 = z2 * z;
+            sq += creal((cexp(z / 2.0) - 1.0) / z);
+            s1 += creal((ez * (z2 - 3.0 * z + 4.0) - z - 4.0) / z3);
+            s2 += creal((ez * (z - 2.0) + z + 2.0) / z3);
+            s3 += creal((ez * (4.0 - z) - z2 - 3.0 * z - 4.0) / z3);
+        }
+        Q[i] = ds * sq / npts;
+        f1[i] = ds * s1 / npts;
+        f2[i] = ds * s2 / npts;
+        f3[i] = ds * s3 / npts;
+    }
+}
+
+/* Etdrk4_PBC::solve (src/Etdrk4_PBC.cc:108-167): the spectrum v of the
+ * current slice is carried from step to step; the nonlinear term is
+ * N(q) = -w q.  Complex-to-complex transforms as the reference. */
+void or_etdrk4_solve(int Lx, int Ly, int Lz, const double *k2neg, double ds, int npts, int len, const double *w,
+                     double *qs) {
+    long M = (long)Lx * Ly * Lz;
+    double *E = malloc(sizeof(double) * M), *E2 = malloc(sizeof(double) * M), *Q = malloc(sizeof(double) * M);
+    double *f1 = malloc(sizeof(double) * M), *f2 = malloc(sizeof(double) * M), *f3 = malloc(sizeof(double) * M);
+    or_etdrk4_coeffs(M, k2neg, ds, npts, E, E2, Q, f1, f2, f3);
+    double *v = malloc(sizeof(double) * 2 * M), *Nv = malloc(sizeof(double) * 2 * M);
+    double *a = malloc(sizeof(double) * 2 * M), *Na = malloc(sizeof(double) * 2 * M);
+    double *Nb = malloc(sizeof(double) * 2 * M), *Nc = malloc(sizeof(double) * 2 * M);
+    double *buf = malloc(sizeof(double) * 2 * M);
+    for (long i = 0; i < M; i++) { v[2 * i] = qs[i]; v[2 * i + 1] = 0.0; }
+    or_fft3d(Lx, Ly, Lz, v, -1);
+    /* N[spec] = F[-w Re(F^-1[spec]) / M] ; N0 from a real slice */
+    #define NONLIN_FROM_REAL(dst, q)                                                   \
+        do {                                                                          \
+            for (long i = 0; i < M; i++) { dst[2 * i] = -w[i] * (q)[i]; dst[2 * i + 1] = 0.0; } \
+            or_fft3d(Lx, Ly, Lz, dst, -1);                                            \
+        } while (0)
+    #define NONLIN_FROM_SPEC(dst, spec)                                                \
+        do {                                                                          \
+            memcpy(buf, spec, sizeof(double) * 2 * M);                                \
+            or_fft3d(Lx, Ly, Lz, buf, +1);                                            \
+            for (long i = 0; i < M; i++) { dst[2 * i] = -w[i] * (buf[2 * i] / (double)M); dst[2 * i + 1] = 0.0; } \
+            or_fft3d(Lx, Ly, Lz, dst, -1);                                            \
+        } while (0)
+    double *b = malloc(sizeof(double) * 2 * M), *c = malloc(sizeof(double) * 2 * M);
+    for (int s = 0; s < len - 1; s++) {
+        NONLIN_FROM_REAL(Nv, qs + (long)s * M);
+        for (long i = 0; i < 2 * M; i++) a[i] = E2[i / 2] * v[i] + Q[i / 2] * Nv[i];
+        NONLIN_FROM_SPEC(Na, a);
+        for (long i = 0; i < 2 * M; i++) b[i] = E2[i / 2] * v[i] + Q[i / 2] * Na[i];
+        NONLIN_FROM_SPEC(Nb, b);
+        for (long i = 0; i < 2 * M; i++) c[i] = E2[i / 2] * a[i] + Q[i / 2] * (2.0 * Nb[i] - Nv[i]);
+        NONLIN_FROM_SPEC(Nc, c);
+        for (long i = 0; i < 2 * M; i++)
+            v[i] = E[i / 2] * v[i] + f1[i / 2] * Nv[i] + 2.0 * f2[i / 2] * (Na[i] + Nb[i]) + f3[i / 2] * Nc[i];
+        memcpy(buf, v, sizeof(double) * 2 * M);
+        or_fft3d(Lx, Ly, Lz, buf, +1);
+        for (long i = 0; i < M; i++) qs[(long)(s + 1) * M + i] = buf[2 * i] / (double)M;
+    }
+    #undef NONLIN_FROM_REAL
+    #undef NONLIN_FROM_SPEC
+    free(E); free(E2); free(Q); free(f1); free(f2); free(f3);
+    free(v); free(Nv); free(a); free(Na); free(Nb); free(Nc); free(buf); free(b); free(c);
+}
+
 /* ----------------------------------------------------------- quadrature */
 
 int or_simpson(int M, int len, const double *q, const double *qc, double cc,
@@ -623,7 +705,7 @@ or_model *or_model_create(int dim, int Lx, int Ly, int Lz,
         free(m);
         return NULL;
     }
-    if (mde == 0) {
+    if (mde == 0 || mde == 2) {
         m->k2neg = (double *)malloc(sizeof(double) * M);
         or_k2_orthogonal(dim, m->l, Lx, Ly, Lz, m->k2neg);
     } else {
@@ -661,6 +743,7 @@ void or_model_destroy(or_model *m) {
 static void solve_prop(or_model *m, double *qs, int len, double ds,
                        const double *lap, const double *w) {
     if (m->mde == 0) or_rqm4_solve(m->Lx, m->Ly, m->Lz, m->k2neg, ds, len, w, qs);
+    else if (m->mde == 2) or_etdrk4_solve(m->Lx, m->Ly, m->Lz, m->k2neg, ds, 32, len, w, qs);
     else or_os2_solve(m->Lx, m->Ly, m->Lz, lap, ds, len, w, qs);
 }
 

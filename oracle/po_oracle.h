@@ -87,10 +87,19 @@ double or_mean(int M, const double *x);
  * Returns 0, or -1 on an exactly zero pivot (x set to 0). */
 int or_lu_solve(int n, double *A, double *b, double *x);
 
+/* ETDRK4 (Cox-Matthews, src/Etdrk4_PBC.cc): coefficient tables for
+ * L = k2neg (M entries each) by the npts-point contour average, and the
+ * propagator solve carrying the spectrum from step to step. */
+void or_etdrk4_coeffs(long M, const double *k2neg, double ds, int npts, double *E, double *E2, double *Q,
+                      double *f1, double *f2, double *f3);
+void or_etdrk4_solve(int Lx, int Ly, int Lz, const double *k2neg, double ds, int npts, int len, const double *w,
+                     double *qs);
+
 /* ---- Model_AB (Anderson / incompressible), src/Model_AB.cc ---------- */
 typedef struct or_model or_model;
 
-/* mde: 0 = RQM4, 1 = OS2.  contour: 0 = Simpson (Quad4_Closed when the
+/* mde: 0 = RQM4, 1 = OS2, 2 = ETDRK4 (32 contour points, the default
+ * Etdrk4_PBC of src/Propagator.cc:115).  contour: 0 = Simpson (Quad4_Closed when the
  * slice count is even, src/Model_AB.cc:719-728), 1 = trapezoid.
  * cell: crystal system code + a,b,c,alpha,beta,gamma (radians).
  * Ms = slice counts [sA, sB] (src/Model_AB.cc:100-111 derives fA and ds).

@@ -31,6 +31,7 @@ extern "C" {
 /* MDE algorithms (AlgorithmMDEType, include/common.h) */
 #define PO_RQM4 0             /* src/RQM4.cc */
 #define PO_OS2 1              /* src/PseudoSpectral.cc */
+#define PO_ETDRK4 2           /* src/Etdrk4_PBC.cc (Cox-Matthews, 32 contour points); no adjoint, no string */
 
 /* contour quadratures */
 #define PO_QUAD_SIMPSON 0     /* src/Simpson.cc */
@@ -68,7 +69,7 @@ typedef struct {
     double chiN;
     double lamA, lamB;
     int n_anderson;
-    int algo;      /* PO_RQM4 or PO_OS2 */
+    int algo;      /* PO_RQM4, PO_OS2 or PO_ETDRK4 */
     int contour;   /* PO_QUAD_AUTO (Simpson rule) or PO_QUAD_TRAP */
 } po_model_desc;
 
@@ -84,9 +85,10 @@ int po_ctx_synchronize(po_ctx *ctx);
 int po_shape_supported(int Lx, int Ly, int Lz);
 
 /* --------------------------------------------------- Propagator + Updater */
-/* Propagator(name, cfg, len, ds, up) with up = new RQM4(uc,Lx,Ly,Lz,ds) or
- * new PseudoSpectral(uc,Lx,Ly,Lz,ds) (src/Propagator.cc:12-38,
- * src/RQM4.cc:9-16, src/PseudoSpectral.cc:33-38).  Slices live on the GPU. */
+/* Propagator(name, cfg, len, ds, up) with up = new RQM4(uc,Lx,Ly,Lz,ds),
+ * new PseudoSpectral(uc,Lx,Ly,Lz,ds) or new Etdrk4_PBC(uc,Lx,Ly,Lz,ds)
+ * (src/Propagator.cc:12-38,107-116, src/RQM4.cc:9-16,
+ * src/PseudoSpectral.cc:33-38, src/Etdrk4_PBC.cc:12-22).  Slices live on the GPU. */
 int po_prop_create(po_ctx *ctx, const po_grid_desc *grid, int algo, int len, double ds, po_prop **out);
 int po_prop_destroy(po_prop *p);
 int po_prop_len(const po_prop *p);

@@ -15,7 +15,17 @@ LIB_PATH = os.path.join(_HERE, "libpolystring_amd.so")
 PO_OK = 0
 PO_ERR_INVALID, PO_ERR_DEVICE, PO_ERR_QUADRATURE, PO_ERR_STATE = 1, 2, 3, 4
 PO_FP64, PO_FP32_BULK = 0, 1
-PO_RQM4, PO_OS2 = 0, 1
+PO_RQM4, PO_OS2, PO_ETDRK4 = 0, 1, 2
+_ALGOS = {"RQM4": PO_RQM4, "OS4": PO_RQM4, "OS2": PO_OS2, "OS": PO_OS2, "PSEUDOSPECTRAL": PO_OS2,
+          "ETDRK4": PO_ETDRK4, "ETDRK": PO_ETDRK4}
+
+
+def _algo(name):
+    """AlgorithmMDEType by name (src/Config.cc:1099-1110); unknown names raise."""
+    try:
+        return _ALGOS[name.upper()]
+    except KeyError:
+        raise ValueError(f"unknown MDE algorithm {name!r}; expected one of {sorted(_ALGOS)}") from None
 PO_QUAD_SIMPSON, PO_QUAD_QUAD4C, PO_QUAD_TRAP, PO_QUAD_AUTO = 0, 1, 2, 3
 FIELDS = {"wA": 0, "wB": 1, "phiA": 2, "phiB": 3}
 
@@ -163,7 +173,7 @@ class Propagator:
         self.ds = ds
         g = grid_desc(shape, cell, dim)
         h = _vp()
-        _check(load().po_prop_create(ctx.h, ctypes.byref(g), PO_RQM4 if algo.upper() == "RQM4" else PO_OS2,
+        _check(load().po_prop_create(ctx.h, ctypes.byref(g), _algo(algo),
                                      length, ds, ctypes.byref(h)))
         self.h = h
 
@@ -222,7 +232,7 @@ class ModelAB:
         self.shape = wA0.shape
         self.ctx = ctx
         d = ModelDesc(grid_desc(self.shape, cell, dim), int(Ms[0]), int(Ms[1]), float(chiN), float(lam[0]),
-                      float(lam[1]), int(n_anderson), PO_RQM4 if algo.upper() == "RQM4" else PO_OS2,
+                      float(lam[1]), int(n_anderson), _algo(algo),
                       PO_QUAD_TRAP if contour.lower() == "trap" else PO_QUAD_AUTO)
         h = _vp()
         _check(load().po_model_create(ctx.h, ctypes.byref(d), _p(wA0), _p(wB0), ctypes.byref(h)))
@@ -341,7 +351,7 @@ class String:
         self.shape = wA.shape[1:]
         self.ctx = ctx
         d = StringDesc(grid_desc(self.shape, cell, dim), int(Ms[0]), int(Ms[1]), float(chiN), float(lam[0]),
-                       float(lam[1]), float(lam[2]), PO_RQM4 if algo.upper() == "RQM4" else PO_OS2,
+                       float(lam[1]), float(lam[2]), _algo(algo),
                        PO_QUAD_TRAP if contour.lower() == "trap" else PO_QUAD_AUTO, int(bool(compressible)),
                        int(bool(mirror)), self.m)
         h = _vp()

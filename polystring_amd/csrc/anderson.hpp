@@ -21,10 +21,12 @@ struct AndersonArgs {
 };
 
 // Optional Boltzmann factor tables refreshed by the mixing kernel:
-// e = exp(c * w_new) with c = -ds/2 and -ds/4 per species.
+// e = exp(c * w_new) with c = -ds/2 and -ds/4 per species.  With linear set,
+// only eA2 / eB2 are written, as c * w_new (the ETDRK4 factor -w).
 template <typename T> struct FactorOut {
     T *eA2, *eA4, *eB2, *eB4;
     double cA2, cA4, cB2, cB4;
+    int linear;
 };
 
 hipError_t launch_anderson_d(const AndersonArgs &a, const double *phiA, const double *phiB, const double *wA,

@@ -48,8 +48,9 @@ namespace po {
 template <typename T> struct Prop : po_prop {
     DBuf<T> q, e2, e4;
     DBuf<double> w, part, scal;
-    DBuf<cx<T>> Sf, Sh;
+    DBuf<cx<T>> Sf, Sh, ev, eacc, et;
     KTable<T> kfull, khalf;
+    EtdTables<T> etd;
     Roots<T> rt{};
     Program<T> prog;
 
@@ -78,6 +79,17 @@ template <typename T> struct Prop : po_prop {
         lanes[0].segs.push_back(Segment<T>{q.p, len, e2.p, e4.p, &kfull, &khalf});
         lanes[0].Sf = Sf.p;
         lanes[0].Sh = Sh.p;
+        if (algo == PO_ETDRK4) {
+            if ((rc = build_etd_tables<T>(etd, sh, cell, ds, st))) return rc;
+            const long ns = etd_slots<T>(sh);
+            PO_HIP(ev.alloc(ns));
+            PO_HIP(eacc.alloc(ns));
+            PO_HIP(et.alloc(ns));
+            lanes[0].segs[0].etd = &etd;
+            lanes[0].ev = ev.p;
+            lanes[0].eacc = eacc.p;
+            lanes[0].et = et.p;
+        }
         prog.slice_elems = sh.M;
         return prog.build(algo, lanes, st);
     }
@@ -93,8 +105,12 @@ template <typename T> struct Prop : po_prop {
     int solve(const double *hw) override {
         hipStream_t st = ctx->stream;
         PO_HIP(hipMemcpyAsync(w.p, hw, sizeof(double) * sh.M, hipMemcpyHostToDevice, st));
-        PO_HIP(launch_exp_factor<T>(w.p, e2.p, -0.5 * ds, sh.M, st));
-        if (algo == PO_RQM4) PO_HIP(launch_exp_factor<T>(w.p, e4.p, -0.25 * ds, sh.M, st));
+        if (algo == PO_ETDRK4) {
+            PO_HIP(launch_scale_field<T>(w.p, e2.p, -1.0, sh.M, st));  // N(q) = -w q
+        } else {
+            PO_HIP(launch_exp_factor<T>(w.p, e2.p, -0.5 * ds, sh.M, st));
+            if (algo == PO_RQM4) PO_HIP(launch_exp_factor<T>(w.p, e4.p, -0.25 * ds, sh.M, st));
+        }
         int rc = prog.run(sh, rt, st);
         if (rc) return rc;
         PO_HIP(hipStreamSynchronize(st));
@@ -174,7 +190,9 @@ template <typename T> struct Model : po_model {
     DBuf<double> wA, wB, wAh, wBh, dAh, dBh, BA, BB, C, phiA, phiB, part, obs;
     DBuf<T> eA2, eA4, eB2, eB4, qA, qB, qAc, qBc;
     DBuf<cx<T>> S[4];
+    DBuf<cx<T>> etdState[6];  // ETDRK4 carried spectra v, acc, t of both lanes
     KTable<T> kA, kAh, kB, kBh;
+    EtdTables<T> etdA, etdB;
     Program<T> prog;
     std::vector<Lane<T>> lanes_;
 
@@ -188,7 +206,8 @@ template <typename T> struct Model : po_model {
         if (d.sA < 2 || d.sB < 2) return fail(PO_ERR_INVALID, "Ms entries must be >= 2");
         if (d.n_anderson < 0 || d.n_anderson > kMaxHist)
             return fail(PO_ERR_INVALID, "n_Anderson_mixing must be in [0, " + std::to_string(kMaxHist) + "]");
-        if (d.algo != PO_RQM4 && d.algo != PO_OS2) return fail(PO_ERR_INVALID, "algo must be RQM4 or OS2");
+        if (d.algo != PO_RQM4 && d.algo != PO_OS2 && d.algo != PO_ETDRK4)
+            return fail(PO_ERR_INVALID, "algo must be RQM4, OS2 or ETDRK4");
         sA = d.sA;
         sB = d.sB;
         n = d.n_anderson;
@@ -249,10 +268,7 @@ template <typename T> struct Model : po_model {
         // heads qA[0] = qBc[0] = 1 (src/Model_AB.cc:779-782)
         PO_HIP(launch_fill<T>(qA.p, (T)1, M, st));
         PO_HIP(launch_fill<T>(qBc.p, (T)1, M, st));
-        PO_HIP(launch_exp_factor<T>(wA.p, eA2.p, -0.5 * dsA, M, st));
-        PO_HIP(launch_exp_factor<T>(wA.p, eA4.p, -0.25 * dsA, M, st));
-        PO_HIP(launch_exp_factor<T>(wB.p, eB2.p, -0.5 * dsB, M, st));
-        PO_HIP(launch_exp_factor<T>(wB.p, eB4.p, -0.25 * dsB, M, st));
+        if ((rc = refresh_factors())) return rc;
         const bool general = (d.algo == PO_OS2) && !cell.ortho;
         if ((rc = build_ktable<T>(kA, sh, cell, general, dsA, st))) return rc;
         if ((rc = build_ktable<T>(kB, sh, cell, general, dsB, st))) return rc;
@@ -268,10 +284,39 @@ template <typename T> struct Model : po_model {
         lanes[1].segs = {Segment<T>{qBc.p, sB, eB2.p, eB4.p, &kB, &kBh}, Segment<T>{qAc.p, sA, eA2.p, eA4.p, &kA, &kAh}};
         lanes[1].Sf = S[2].p;
         lanes[1].Sh = S[3].p;
+        if (d.algo == PO_ETDRK4) {
+            if ((rc = build_etd_tables<T>(etdA, sh, cell, dsA, st))) return rc;
+            if ((rc = build_etd_tables<T>(etdB, sh, cell, dsB, st))) return rc;
+            const long ns = etd_slots<T>(sh);
+            for (auto &b : etdState) PO_HIP(b.alloc(ns));
+            for (int l = 0; l < 2; ++l) {
+                for (auto &sg : lanes[l].segs) sg.etd = sg.q == qA.p || sg.q == qAc.p ? &etdA : &etdB;
+                lanes[l].ev = etdState[3 * l].p;
+                lanes[l].eacc = etdState[3 * l + 1].p;
+                lanes[l].et = etdState[3 * l + 2].p;
+            }
+        }
         prog.slice_elems = M;
         lanes_ = lanes;
         if ((rc = prog.build(d.algo, lanes, st))) return rc;
         return prog.tune(sh, rt, st);
+    }
+
+    // Propagator factor tables of the current fields: exp(-w ds/2), exp(-w ds/4)
+    // (RQM4 / OS2), or -w (ETDRK4's nonlinear term, src/Etdrk4_PBC.cc:136)
+    int refresh_factors() {
+        hipStream_t st = ctx->stream;
+        const long M = sh.M;
+        if (desc.algo == PO_ETDRK4) {
+            PO_HIP(launch_scale_field<T>(wA.p, eA2.p, -1.0, M, st));
+            PO_HIP(launch_scale_field<T>(wB.p, eB2.p, -1.0, M, st));
+            return PO_OK;
+        }
+        PO_HIP(launch_exp_factor<T>(wA.p, eA2.p, -0.5 * dsA, M, st));
+        PO_HIP(launch_exp_factor<T>(wA.p, eA4.p, -0.25 * dsA, M, st));
+        PO_HIP(launch_exp_factor<T>(wB.p, eB2.p, -0.5 * dsB, M, st));
+        PO_HIP(launch_exp_factor<T>(wB.p, eB4.p, -0.25 * dsB, M, st));
+        return PO_OK;
     }
 
     int find_index(int j) const {  // src/FieldAX.cc:262-267
@@ -318,7 +363,8 @@ template <typename T> struct Model : po_model {
             a.lamB = 1.0 - std::pow(desc.lamB, (double)(k + 1));
             PO_HIP(launch_anderson_d(a, phiA.p, phiB.p, wA.p, wB.p, part.p, st));
             PO_HIP(launch_anderson_solve(a, part.p, BA.p, BB.p, C.p, obs.p, st));
-            FactorOut<T> fo{eA2.p, eA4.p, eB2.p, eB4.p, -0.5 * dsA, -0.25 * dsA, -0.5 * dsB, -0.25 * dsB};
+            FactorOut<T> fo{eA2.p, eA4.p, eB2.p, eB4.p, -0.5 * dsA, -0.25 * dsA, -0.5 * dsB, -0.25 * dsB, 0};
+            if (desc.algo == PO_ETDRK4) fo = FactorOut<T>{eA2.p, nullptr, eB2.p, nullptr, -1.0, 0.0, -1.0, 0.0, 1};
             PO_HIP(launch_anderson_mix<T>(a, C.p, wA.p, wB.p, phiA.p, phiB.p, fo, part.p, st));
             PO_HIP(launch_observables(part.p, M, obs.p, st));
             PO_HIP(mark());
@@ -363,6 +409,7 @@ template <typename T> struct Model : po_model {
             fused_on = false;
             return PO_OK;
         }
+        if (desc.algo == PO_ETDRK4) return fail(PO_ERR_INVALID, "ETDRK4 has no adjoint");
         if (!prog_adj.seq.empty()) {
             fused_on = true;
             return PO_OK;
@@ -422,6 +469,7 @@ template <typename T> struct Model : po_model {
 
     int adj_build() {
         if (adj_ready) return PO_OK;
+        if (desc.algo == PO_ETDRK4) return fail(PO_ERR_INVALID, "ETDRK4 has no adjoint (stress)");
         const bool rqm4 = desc.algo == PO_RQM4;
         if (!rqm4 && !cell.ortho)
             return fail(PO_ERR_INVALID, "stress needs separable k tables (RQM4, or OS2 on an orthogonal cell)");
@@ -564,6 +612,7 @@ template <typename T> struct Model : po_model {
 
     int grad_build() {
         if (g_ready) return PO_OK;
+        if (desc.algo == PO_ETDRK4) return fail(PO_ERR_INVALID, "ETDRK4 has no adjoint (field gradient)");
         const bool rqm4 = desc.algo == PO_RQM4;
         hipStream_t st = ctx->stream;
         const int nA = sA - 1, nB = sB - 1, N = nA + nB;
@@ -693,10 +742,8 @@ template <typename T> struct Model : po_model {
         PO_HIP(hipStreamSynchronize(st));
         PO_HIP(hipMemcpyAsync(wA.p, hA, sizeof(double) * M, hipMemcpyHostToDevice, st));
         PO_HIP(hipMemcpyAsync(wB.p, hB, sizeof(double) * M, hipMemcpyHostToDevice, st));
-        PO_HIP(launch_exp_factor<T>(wA.p, eA2.p, -0.5 * dsA, M, st));
-        PO_HIP(launch_exp_factor<T>(wA.p, eA4.p, -0.25 * dsA, M, st));
-        PO_HIP(launch_exp_factor<T>(wB.p, eB2.p, -0.5 * dsB, M, st));
-        PO_HIP(launch_exp_factor<T>(wB.p, eB4.p, -0.25 * dsB, M, st));
+        int rc = refresh_factors();
+        if (rc) return rc;
         return restart_history();
     }
 
@@ -738,6 +785,10 @@ template <typename T> struct Model : po_model {
         if (desc.algo == PO_RQM4) {
             if ((rc = build_ktable<T>(kAh, sh, cell, false, 0.5 * dsA, st))) return rc;
             if ((rc = build_ktable<T>(kBh, sh, cell, false, 0.5 * dsB, st))) return rc;
+        }
+        if (desc.algo == PO_ETDRK4) {  // refilled in place: the program keeps its descriptors
+            if ((rc = build_etd_tables<T>(etdA, sh, cell, dsA, st))) return rc;
+            if ((rc = build_etd_tables<T>(etdB, sh, cell, dsB, st))) return rc;
         }
         return restart_history();
     }
@@ -803,7 +854,8 @@ int po_shape_supported(int Lx, int Ly, int Lz) { return po::plane_supported(Ly, 
 
 int po_prop_create(po_ctx *ctx, const po_grid_desc *grid, int algo, int len, double ds, po_prop **out) {
     if (!ctx || !grid || !out) return fail(PO_ERR_INVALID, "null argument");
-    if (algo != PO_RQM4 && algo != PO_OS2) return fail(PO_ERR_INVALID, "algo must be PO_RQM4 or PO_OS2");
+    if (algo != PO_RQM4 && algo != PO_OS2 && algo != PO_ETDRK4)
+        return fail(PO_ERR_INVALID, "algo must be PO_RQM4, PO_OS2 or PO_ETDRK4");
     if (len < 2) return fail(PO_ERR_INVALID, "len must be >= 2");
     PO_HIP(hipSetDevice(ctx->device));
     po_prop *p;

@@ -73,6 +73,26 @@ template <typename T> struct DotDesc {
     double *partials;  // [xdot_blocks][3]
 };
 
+// Per-lane descriptor of the ETDRK4 x-pencil kernel (src/Etdrk4_PBC.cc:108-167).
+// S holds on entry the yz spectrum of the nonlinear term -w q of the current
+// stage; the kernel completes its x transform to N, advances the carried
+// spectra, and leaves in S the x-inverse of the next stage's input spectrum
+// (a, b, c or the new v).  Stages 0..3 (A..D), all spectra unnormalised:
+//   A: a = E2 v + Q Nv;          acc = E v + f1 Nv;  t = E2 a - Q Nv;  out a
+//   B: b = E2 v + Q Na;          acc += 2 f2 Na;                       out b
+//   C: c = t + 2 Q Nb            (= E2 a + Q (2 Nb - Nv));  acc += 2 f2 Nb;  out c
+//   D: v = acc + f3 Nc           (= E v + f1 Nv + 2 f2 (Na + Nb) + f3 Nc); out v
+// At the head of a segment stage A first sets v = F q_0 from Sinit (the yz
+// spectrum of q_0).  v, acc, t and the coefficient tables are in slot order
+// (etd_slots): the register order of the kernel on full tiles, else x-major
+// natural order.
+template <typename T> struct EtdDesc {
+    cx<T> *S;
+    const cx<T> *Sinit;
+    cx<T> *v, *acc, *t;
+    const T *E, *E2, *Q, *f1, *f2, *f3;
+};
+
 struct Shape {
     int Lx, Ly, Lz, Hz;
     long M, Mh;
@@ -119,9 +139,22 @@ hipError_t launch_xpass(const Shape &sh, const Roots<T> &rt, const XDesc<T> *d_d
                         hipStream_t st, int xvariant = 0, int split = 0);
 bool split_supported(const Shape &sh);
 
+// ETDRK4 x-pencil stage (see EtdDesc); stage in [0, 4).
+template <typename T>
+hipError_t launch_xetd(const Shape &sh, const Roots<T> &rt, const EtdDesc<T> *d_desc, int nlanes, int stage,
+                       hipStream_t st);
+// complex slots of one ETDRK4 spectrum / coefficient table
+template <typename T> long etd_slots(const Shape &sh);
+// dst[slot] = src[x * ncol + col] over the slots of the ETDRK4 kernel
+// (src: x-major natural order of Mh entries)
+template <typename T> hipError_t launch_etd_gather(const Shape &sh, const T *src, T *dst, hipStream_t st);
+
 // Elementwise / reduction launchers (kernels.hip).
 template <typename T>
 hipError_t launch_exp_factor(const double *w, T *out, double coef, long M, hipStream_t st);
+// out = coef * w (the ETDRK4 nonlinear factor -w)
+template <typename T>
+hipError_t launch_scale_field(const double *w, T *out, double coef, long M, hipStream_t st);
 template <typename T>
 hipError_t launch_fill(T *p, T v, long n, hipStream_t st);
 

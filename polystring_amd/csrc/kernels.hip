@@ -46,6 +46,18 @@ hipError_t launch_exp_factor(const double *w, T *out, double coef, long M, hipSt
     return hipGetLastError();
 }
 
+template <typename T>
+__global__ void k_scale_field(const double *__restrict__ w, T *__restrict__ out, double coef, long M) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < M; i += (long)gridDim.x * blockDim.x)
+        out[i] = (T)(coef * w[i]);
+}
+
+template <typename T>
+hipError_t launch_scale_field(const double *w, T *out, double coef, long M, hipStream_t st) {
+    hipLaunchKernelGGL(k_scale_field<T>, dim3(grid_for(M, 256, 8192)), dim3(256), 0, st, w, out, coef, M);
+    return hipGetLastError();
+}
+
 template <typename T> __global__ void k_fill(T *p, T v, long n) {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = v;
 }
@@ -365,7 +377,10 @@ k_anderson_mix(AndersonArgs a, const double *__restrict__ C, double *__restrict_
         wBn[i] = nb;
         wA[i] = na;
         wB[i] = nb;
-        if (fo.eA2) {
+        if (fo.eA2 && fo.linear) {
+            fo.eA2[i] = (T)(fo.cA2 * na);
+            fo.eB2[i] = (T)(fo.cB2 * nb);
+        } else if (fo.eA2) {
             fo.eA2[i] = (T)exp(fo.cA2 * na);
             fo.eA4[i] = (T)exp(fo.cA4 * na);
             fo.eB2[i] = (T)exp(fo.cB2 * nb);
@@ -431,6 +446,7 @@ hipError_t launch_observables(const double *partials, long M, double *obs, hipSt
 
 #define PO_INST(T)                                                                                              \
     template hipError_t launch_exp_factor<T>(const double *, T *, double, long, hipStream_t);                   \
+    template hipError_t launch_scale_field<T>(const double *, T *, double, long, hipStream_t);                  \
     template hipError_t launch_fill<T>(T *, T, long, hipStream_t);                                              \
     template hipError_t launch_density<T>(const T *, const T *, int, long, int, const double *, double, double, \
                                           double *, hipStream_t);                                               \

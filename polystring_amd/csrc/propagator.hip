@@ -872,6 +872,155 @@ k_xdot3(const DotDesc<T> *__restrict__ desc, Roots<T> rt, int ncol, int hz, int 
     }
 }
 
+// ---------------------------------------------------------------- ETDRK4 --
+// One point of an ETDRK4 stage (EtdDesc): N is the full spectrum of the
+// stage's nonlinear term, the return value the next stage's input spectrum.
+template <typename T>
+__device__ __forceinline__ cx<T> etd_point(int stage, const EtdDesc<T> &d, unsigned slot, cx<T> N) {
+    const T two = (T)2;
+    if (stage == 0) {
+        const cx<T> v = gload(d.v + slot);
+        const T e = gload(d.E + slot), e2 = gload(d.E2 + slot), q = gload(d.Q + slot), f1 = gload(d.f1 + slot);
+        const cx<T> a = mk<T>(e2 * v.x + q * N.x, e2 * v.y + q * N.y);
+        gstore(d.acc + slot, mk<T>(e * v.x + f1 * N.x, e * v.y + f1 * N.y));
+        gstore(d.t + slot, mk<T>(e2 * a.x - q * N.x, e2 * a.y - q * N.y));
+        return a;
+    }
+    if (stage == 1) {
+        const cx<T> v = gload(d.v + slot), acc = gload(d.acc + slot);
+        const T e2 = gload(d.E2 + slot), q = gload(d.Q + slot), f2 = two * gload(d.f2 + slot);
+        gstore(d.acc + slot, mk<T>(acc.x + f2 * N.x, acc.y + f2 * N.y));
+        return mk<T>(e2 * v.x + q * N.x, e2 * v.y + q * N.y);
+    }
+    if (stage == 2) {
+        const cx<T> t = gload(d.t + slot), acc = gload(d.acc + slot);
+        const T q = two * gload(d.Q + slot), f2 = two * gload(d.f2 + slot);
+        gstore(d.acc + slot, mk<T>(acc.x + f2 * N.x, acc.y + f2 * N.y));
+        return mk<T>(t.x + q * N.x, t.y + q * N.y);
+    }
+    const cx<T> acc = gload(d.acc + slot);
+    const T f3 = gload(d.f3 + slot);
+    const cx<T> v = mk<T>(acc.x + f3 * N.x, acc.y + f3 * N.y);
+    gstore(d.v + slot, v);
+    return v;
+}
+
+// ETDRK4 x-pencil kernel on full 16-column tiles: the k_xpass3 (W = 0)
+// structure with the stage combine in registers; slots are register order,
+// slot = sbase(block, wave) + (it * RX + r) * 64 + lane.
+template <typename T, int LX>
+__global__ void __launch_bounds__((X3Cfg<T, LX>::NT))
+k_xetd3(const EtdDesc<T> *__restrict__ descs, Roots<T> rt, int ncol, int stage) {
+    using X = X3Cfg<T, LX>;
+    constexpr int CPW = X::CPW, RS = X::RS;
+    using F = WaveEnds<LX, false, CPW>;
+    using I = WaveEnds<LX, true, CPW>;
+    static_assert(F::RL == I::RF && F::ITL == I::ITF, "x transform ends must match");
+    constexpr int RX = F::RL, ITX = F::ITL, NBX = LX / RX, TOTX = CPW * NBX;
+    (void)NBX;
+    __shared__ cx<T> buf[LX * RS];
+    const EtdDesc<T> d = descs[blockIdx.y];
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int cw = blockIdx.x * X::C + w * CPW;
+    cx<T> *lds = buf + w * CPW;
+    const unsigned sbase = (blockIdx.x * X::NW + w) * (unsigned)(ITX * RX * 64);
+    cx<T> v[ITX][RX];
+    if (d.Sinit) {
+        {
+            cx<T> a[F::ITF][F::RF];
+            WaveFFT<T, LX, false, CPW, true, IO_GLB, IO_REG>::run(lds, 1, RS, rt.x, lane, d.Sinit + cw, 1, ncol,
+                                                                  nullptr, a, v);
+        }
+#pragma unroll
+        for (int it = 0; it < ITX; ++it)
+            if ((TOTX % 64 == 0) || lane + it * 64 < TOTX)
+#pragma unroll
+                for (int r = 0; r < RX; ++r) gstore(d.v + sbase + (unsigned)((it * RX + r) * 64 + lane), v[it][r]);
+    }
+    {
+        cx<T> a[F::ITF][F::RF];
+        WaveFFT<T, LX, false, CPW, true, IO_GLB, IO_REG>::run(lds, 1, RS, rt.x, lane, d.S + cw, 1, ncol, nullptr, a,
+                                                              v);
+    }
+#pragma unroll
+    for (int it = 0; it < ITX; ++it)
+        if ((TOTX % 64 == 0) || lane + it * 64 < TOTX)
+#pragma unroll
+            for (int r = 0; r < RX; ++r)
+                v[it][r] = etd_point<T>(stage, d, sbase + (unsigned)((it * RX + r) * 64 + lane), v[it][r]);
+    {
+        cx<T> b[I::ITL][I::RL];
+        WaveFFT<T, LX, true, CPW, true, IO_REG, IO_GLB>::run(lds, 1, RS, rt.x, lane, nullptr, 1, ncol, d.S + cw, v, b);
+    }
+}
+
+// ETDRK4 x-pencil kernel for any column count (1-D and small 2-D grids):
+// tiles of C columns in LDS, slots in natural order x * ncol + col.
+template <typename T, int LX>
+__global__ void __launch_bounds__((XCfg<T, LX>::NT))
+k_xetd(const EtdDesc<T> *__restrict__ descs, Roots<T> rt, int ncol, int stage) {
+    constexpr int C = XCfg<T, LX>::C, NT = XCfg<T, LX>::NT;
+    __shared__ cx<T> buf[LX * C];
+    const EtdDesc<T> d = descs[blockIdx.y];
+    const int c0 = blockIdx.x * C;
+    const int tid = threadIdx.x;
+    if (d.Sinit) {
+        for (int idx = tid; idx < LX * C; idx += NT) {
+            const int x = idx / C, col = c0 + idx % C;
+            buf[idx] = col < ncol ? d.Sinit[(long)x * ncol + col] : mk<T>((T)0, (T)0);
+        }
+        __syncthreads();
+        fft_lds<T, LX, false, NT, C, true>(buf, 1, C, rt.x);
+        for (int idx = tid; idx < LX * C; idx += NT) {
+            const int x = idx / C, col = c0 + idx % C;
+            if (col < ncol) d.v[(long)x * ncol + col] = buf[idx];
+        }
+        __syncthreads();
+    }
+    for (int idx = tid; idx < LX * C; idx += NT) {
+        const int x = idx / C, col = c0 + idx % C;
+        buf[idx] = col < ncol ? d.S[(long)x * ncol + col] : mk<T>((T)0, (T)0);
+    }
+    __syncthreads();
+    fft_lds<T, LX, false, NT, C, true>(buf, 1, C, rt.x);
+    for (int idx = tid; idx < LX * C; idx += NT) {
+        const int x = idx / C, col = c0 + idx % C;
+        if (col < ncol) buf[idx] = etd_point<T>(stage, d, (unsigned)(x * ncol + col), buf[idx]);
+    }
+    __syncthreads();
+    fft_lds<T, LX, true, NT, C, true>(buf, 1, C, rt.x);
+    for (int idx = tid; idx < LX * C; idx += NT) {
+        const int x = idx / C, col = c0 + idx % C;
+        if (col < ncol) d.S[(long)x * ncol + col] = buf[idx];
+    }
+}
+
+// Natural-order table to the register-order slots of k_xetd3 (same
+// block / wave / lane geometry, no transform).  Padding slots get 0.
+template <typename T, int LX>
+__global__ void __launch_bounds__((X3Cfg<T, LX>::NT))
+k_etd_gather3(const T *__restrict__ src, T *__restrict__ dst, int ncol) {
+    using X = X3Cfg<T, LX>;
+    constexpr int CPW = X::CPW;
+    using F = WaveEnds<LX, false, CPW>;
+    constexpr int RX = F::RL, ITX = F::ITL, NBX = LX / RX, TOTX = CPW * NBX;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int cw = blockIdx.x * X::C + w * CPW;
+    const unsigned sbase = (blockIdx.x * X::NW + w) * (unsigned)(ITX * RX * 64);
+    for (int it = 0; it < ITX; ++it) {
+        const int b = lane + it * 64;
+        for (int r = 0; r < RX; ++r) {
+            const unsigned slot = sbase + (unsigned)((it * RX + r) * 64 + lane);
+            if ((TOTX % 64 == 0) || b < TOTX) {
+                const int c = cw + b % CPW, x = b / CPW + r * NBX;
+                dst[slot] = src[(long)x * ncol + c];
+            } else {
+                dst[slot] = (T)0;
+            }
+        }
+    }
+}
+
 // ------------------------------------------------------------- dispatch --
 #define PO_PLANE_SHAPES(X) \
     X(1, 1) X(8, 1) X(16, 1) X(32, 1) X(48, 1) X(64, 1) X(96, 1) X(128, 1) X(256, 1) \
@@ -1044,6 +1193,57 @@ template hipError_t launch_xdot<double>(const Shape &, const Roots<double> &, co
                                         hipStream_t);
 template hipError_t launch_xdot<float>(const Shape &, const Roots<float> &, const DotDesc<float> *, int,
                                        hipStream_t);
+
+template <typename T>
+hipError_t launch_xetd(const Shape &sh, const Roots<T> &rt, const EtdDesc<T> *d_desc, int nlanes, int stage,
+                       hipStream_t st) {
+    const int ncol = sh.Ly * sh.Hz;
+    if (stage < 0 || stage > 3) return hipErrorInvalidValue;
+#define PO_CASE(a)                                                                                          \
+    if (sh.Lx == a) {                                                                                       \
+        if (ncol % X3Cfg<T, a>::C == 0)                                                                     \
+            hipLaunchKernelGGL((k_xetd3<T, a>), dim3(ncol / X3Cfg<T, a>::C, nlanes), dim3(X3Cfg<T, a>::NT), 0, st, \
+                               d_desc, rt, ncol, stage);                                                    \
+        else                                                                                                \
+            hipLaunchKernelGGL((k_xetd<T, a>), dim3((ncol + XCfg<T, a>::C - 1) / XCfg<T, a>::C, nlanes),       \
+                               dim3(XCfg<T, a>::NT), 0, st, d_desc, rt, ncol, stage);                      \
+        return hipGetLastError();                                                                           \
+    }
+    PO_X_SIZES(PO_CASE)
+#undef PO_CASE
+    return hipErrorInvalidValue;
+}
+
+template <typename T> long etd_slots(const Shape &sh) {
+    const int ncol = sh.Ly * sh.Hz;
+    if (ncol % 16 != 0) return sh.Mh;
+    return xstore_elems<T>(sh, 0, 0);
+}
+
+template <typename T> hipError_t launch_etd_gather(const Shape &sh, const T *src, T *dst, hipStream_t st) {
+    const int ncol = sh.Ly * sh.Hz;
+    if (ncol % 16 != 0)
+        return hipMemcpyAsync(dst, src, sizeof(T) * sh.Mh, hipMemcpyDeviceToDevice, st);
+#define PO_CASE(a)                                                                                          \
+    if (sh.Lx == a) {                                                                                       \
+        static_assert(X3Cfg<T, a>::C == 16, "etd_slots assumes 16-column tiles");                           \
+        hipLaunchKernelGGL((k_etd_gather3<T, a>), dim3(ncol / X3Cfg<T, a>::C), dim3(X3Cfg<T, a>::NT), 0, st, src, \
+                           dst, ncol);                                                                      \
+        return hipGetLastError();                                                                           \
+    }
+    PO_X_SIZES(PO_CASE)
+#undef PO_CASE
+    return hipErrorInvalidValue;
+}
+
+template hipError_t launch_xetd<double>(const Shape &, const Roots<double> &, const EtdDesc<double> *, int, int,
+                                        hipStream_t);
+template hipError_t launch_xetd<float>(const Shape &, const Roots<float> &, const EtdDesc<float> *, int, int,
+                                       hipStream_t);
+template long etd_slots<double>(const Shape &);
+template long etd_slots<float>(const Shape &);
+template hipError_t launch_etd_gather<double>(const Shape &, const double *, double *, hipStream_t);
+template hipError_t launch_etd_gather<float>(const Shape &, const float *, float *, hipStream_t);
 
 template hipError_t launch_plane<double>(const Shape &, const Roots<double> &, const PlaneDesc<double> *, int,
                                          hipStream_t, int, int);

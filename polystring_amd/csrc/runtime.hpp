@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <complex>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -277,17 +278,79 @@ inline int make_shape(const po_grid_desc &g, Shape &sh) {
 // of each segment being the tail of the previous one (qB after qA, qAc after
 // qBc: src/Model_AB.cc:139-145).  Lanes of one program advance in lockstep,
 // one launch per pass for all lanes.
+template <typename T> struct EtdTables;
+
 template <typename T> struct Segment {
     T *q;
     int len;
-    const T *e2, *e4;            // exp(-w ds/2), exp(-w ds/4)
+    const T *e2, *e4;            // exp(-w ds/2), exp(-w ds/4); ETDRK4: e2 = -w
     const KTable<T> *kfull, *khalf;  // K for h = ds and h = ds/2
+    const EtdTables<T> *etd = nullptr;  // ETDRK4 coefficients for this ds
 };
 
 template <typename T> struct Lane {
     std::vector<Segment<T>> segs;
     cx<T> *Sf, *Sh;
+    cx<T> *ev = nullptr, *eacc = nullptr, *et = nullptr;  // ETDRK4 carried spectra (etd_slots each)
 };
+
+// ETDRK4 coefficients of Etdrk4_PBC::init_coefficient (src/Etdrk4_PBC.cc:43-90)
+// for L = -|k|^2 of the orthogonal cell (UnitCell::calc_k2_orthogonal,
+// src/UnitCell.cc:368-418): E = e^{ds L}, E2 = e^{ds L/2}, and Q, f1, f2, f3 as
+// means of Re phi(ds L + r_j) over the npts = 32 points r_j = exp(i pi (j + 1/2)
+// / npts) of the upper unit semicircle (Kassam-Trefethen), with
+//   Q  = ds (e^{z/2} - 1) / z,               f1 = ds (e^z (z^2 - 3z + 4) - z - 4) / z^3,
+//   f2 = ds (e^z (z - 2) + z + 2) / z^3,     f3 = ds (e^z (4 - z) - z^2 - 3z - 4) / z^3.
+// Evaluated on the host in double (std::complex), then placed in the slot
+// order of the ETDRK4 x-pencil kernel.
+template <typename T> struct EtdTables {
+    DBuf<T> E, E2, Q, f1, f2, f3;
+};
+
+template <typename T>
+int build_etd_tables(EtdTables<T> &et, const Shape &sh, const Cell &cell, double ds, hipStream_t st) {
+    const double pi = 3.14159265358979323846264338327950288;
+    const int npts = 32;
+    double cc[3] = {0, 0, 0};
+    for (int d = 0; d < cell.dim; ++d) cc[d] = -4.0 * pi * pi / (cell.l[d] * cell.l[d]);
+    const long n = sh.Mh;
+    std::vector<std::vector<T>> h(6, std::vector<T>((size_t)n));
+    std::vector<std::complex<double>> r(npts);
+    for (int j = 0; j < npts; ++j) r[j] = std::exp(std::complex<double>(0.0, pi * (j + 0.5) / npts));
+    for (int i = 0; i < sh.Lx; ++i)
+        for (int j = 0; j < sh.Ly; ++j)
+            for (int k = 0; k < sh.Hz; ++k) {
+                const double ri = rmin(i, sh.Lx), rj = rmin(j, sh.Ly), rk = rmin(k, sh.Lz);
+                const double L = cc[0] * ri * ri + cc[1] * rj * rj + cc[2] * rk * rk;
+                double sq = 0, s1 = 0, s2 = 0, s3 = 0;
+                for (int m = 0; m < npts; ++m) {
+                    const std::complex<double> z = ds * L + r[m], ez = std::exp(z), z2 = z * z, z3 = z2 * z;
+                    sq += std::real((std::exp(z / 2.0) - 1.0) / z);
+                    s1 += std::real((ez * (z2 - 3.0 * z + 4.0) - z - 4.0) / z3);
+                    s2 += std::real((ez * (z - 2.0) + z + 2.0) / z3);
+                    s3 += std::real((ez * (4.0 - z) - z2 - 3.0 * z - 4.0) / z3);
+                }
+                const size_t o = ((size_t)i * sh.Ly + j) * sh.Hz + k;
+                h[0][o] = (T)std::exp(ds * L);
+                h[1][o] = (T)std::exp(ds * L / 2.0);
+                h[2][o] = (T)(ds * sq / npts);
+                h[3][o] = (T)(ds * s1 / npts);
+                h[4][o] = (T)(ds * s2 / npts);
+                h[5][o] = (T)(ds * s3 / npts);
+            }
+    const long slots = etd_slots<T>(sh);
+    if (slots < n) return fail(PO_ERR_INVALID, "grid has no ETDRK4 x-pencil layout");
+    DBuf<T> tmp;
+    PO_HIP(tmp.alloc(n));
+    DBuf<T> *dst[6] = {&et.E, &et.E2, &et.Q, &et.f1, &et.f2, &et.f3};
+    for (int c = 0; c < 6; ++c) {
+        PO_HIP(ensure(*dst[c], (size_t)slots));
+        PO_HIP(hipMemcpyAsync(tmp.p, h[c].data(), sizeof(T) * n, hipMemcpyHostToDevice, st));
+        PO_HIP(launch_etd_gather<T>(sh, tmp.p, dst[c]->p, st));
+        PO_HIP(hipStreamSynchronize(st));
+    }
+    return PO_OK;
+}
 
 // Fused adjoint stress of a two-lane program (lane 0 the forward chain,
 // lane 1 the complementary chain, both N steps with N even).  Step s of the
@@ -311,13 +374,21 @@ template <typename T> struct Program {
     struct Launch {
         bool plane;
         int off, n;
+        int stage = -1;  // >= 0: ETDRK4 x-pencil stage (descriptors in ed)
     };
     DBuf<PlaneDesc<T>> pd;
     DBuf<XDesc<T>> xd;
+    DBuf<EtdDesc<T>> ed;
     std::vector<Launch> seq;
     int ndot = 0;  // dot descriptors of a fused adjoint program
+    bool etd = false;  // an ETDRK4 program (unsplit layout, 16-column x tiles)
 
     int build(int algo, const std::vector<Lane<T>> &lanes, hipStream_t st, const AdjointArena<T> *adj = nullptr) {
+        if (algo == PO_ETDRK4) {
+            if (adj) return fail(PO_ERR_INVALID, "ETDRK4 has no adjoint");
+            return build_etd(lanes, st);
+        }
+        etd = false;
         const int L = (int)lanes.size();
         int nsteps = -1;
         for (const auto &ln : lanes) {
@@ -430,10 +501,112 @@ template <typename T> struct Program {
         return PO_OK;
     }
 
+    // ETDRK4 (src/Etdrk4_PBC.cc:108-167): per contour step four x-pencil
+    // stages (EtdDesc), each followed by a yz-plane pass that completes the
+    // inverse transform, multiplies by -w (pre0) and starts the forward
+    // transform of the next nonlinear term.  The plane pass after stage D
+    // also stores the slice q_{s+1}; at the end of a segment it writes the
+    // next segment's head and the yz spectrum of q (out1) from which the next
+    // stage A restarts the carried spectrum v = F q_0, as the reference's
+    // separate solve of the next block does.
+    int build_etd(const std::vector<Lane<T>> &lanes, hipStream_t st) {
+        etd = true;
+        const int L = (int)lanes.size();
+        int nsteps = -1;
+        for (const auto &ln : lanes) {
+            int s = 0;
+            if (!ln.ev || !ln.eacc || !ln.et) return fail(PO_ERR_INVALID, "ETDRK4 lane without state spectra");
+            for (const auto &sg : ln.segs) {
+                if (sg.len < 2) return fail(PO_ERR_INVALID, "propagator length must be >= 2");
+                if (!sg.etd) return fail(PO_ERR_INVALID, "ETDRK4 segment without coefficient tables");
+                s += sg.len - 1;
+            }
+            if (nsteps >= 0 && s != nsteps) return fail(PO_ERR_INVALID, "lockstep lanes need equal step counts");
+            nsteps = s;
+        }
+        std::vector<PlaneDesc<T>> P;
+        std::vector<EtdDesc<T>> E;
+        seq.clear();
+        auto at = [&](int lane, int t, int &si, int &s) {
+            si = 0;
+            s = t;
+            while (s >= lanes[lane].segs[si].len - 1) { s -= lanes[lane].segs[si].len - 1; ++si; }
+        };
+        {
+            std::vector<PlaneDesc<T>> v;
+            for (int l = 0; l < L; ++l) {
+                const auto &sg = lanes[l].segs[0];
+                PlaneDesc<T> d{};
+                d.qin = sg.q;
+                d.out0 = lanes[l].Sf;
+                d.pre0 = sg.e2;
+                d.out1 = lanes[l].Sh;
+                v.push_back(d);
+            }
+            seq.push_back({true, (int)P.size(), (int)v.size()});
+            P.insert(P.end(), v.begin(), v.end());
+        }
+        for (int t = 0; t < nsteps; ++t) {
+            for (int stage = 0; stage < 4; ++stage) {
+                std::vector<EtdDesc<T>> ex;
+                std::vector<PlaneDesc<T>> pl;
+                for (int l = 0; l < L; ++l) {
+                    int si, s;
+                    at(l, t, si, s);
+                    const auto &sg = lanes[l].segs[si];
+                    const auto &tb = *sg.etd;
+                    EtdDesc<T> e{};
+                    e.S = lanes[l].Sf;
+                    e.Sinit = (stage == 0 && s == 0) ? lanes[l].Sh : nullptr;
+                    e.v = lanes[l].ev;
+                    e.acc = lanes[l].eacc;
+                    e.t = lanes[l].et;
+                    e.E = tb.E.p;
+                    e.E2 = tb.E2.p;
+                    e.Q = tb.Q.p;
+                    e.f1 = tb.f1.p;
+                    e.f2 = tb.f2.p;
+                    e.f3 = tb.f3.p;
+                    ex.push_back(e);
+                    PlaneDesc<T> p{};
+                    p.in0 = lanes[l].Sf;
+                    if (stage < 3) {
+                        p.out0 = lanes[l].Sf;
+                        p.pre0 = sg.e2;
+                    } else {
+                        p.qout = sg.q + (size_t)(s + 1) * slice_elems;
+                        const bool last = s + 1 == sg.len - 1;
+                        if (last && si + 1 < (int)lanes[l].segs.size()) p.qout2 = lanes[l].segs[si + 1].q;
+                        if (t + 1 < nsteps) {
+                            int sn, ssn;
+                            at(l, t + 1, sn, ssn);
+                            p.out0 = lanes[l].Sf;
+                            p.pre0 = lanes[l].segs[sn].e2;
+                            if (ssn == 0) p.out1 = lanes[l].Sh;
+                        }
+                    }
+                    pl.push_back(p);
+                }
+                seq.push_back({false, (int)E.size(), (int)ex.size(), stage});
+                E.insert(E.end(), ex.begin(), ex.end());
+                seq.push_back({true, (int)P.size(), (int)pl.size()});
+                P.insert(P.end(), pl.begin(), pl.end());
+            }
+        }
+        has_tab = false;
+        PO_HIP(pd.alloc(P.size()));
+        PO_HIP(ed.alloc(E.size()));
+        PO_HIP(hipMemcpyAsync(pd.p, P.data(), sizeof(PlaneDesc<T>) * P.size(), hipMemcpyHostToDevice, st));
+        PO_HIP(hipMemcpyAsync(ed.p, E.data(), sizeof(EtdDesc<T>) * E.size(), hipMemcpyHostToDevice, st));
+        PO_HIP(hipStreamSynchronize(st));
+        return PO_OK;
+    }
+
     int run(const Shape &sh, const Roots<T> &rt, hipStream_t st) const {
         for (const auto &l : seq) {
-            hipError_t e = l.plane ? launch_plane<T>(sh, rt, pd.p + l.off, l.n, st, plane_variant, split)
-                                   : launch_xpass<T>(sh, rt, xd.p + l.off, l.n, st, x_variant, split);
+            hipError_t e = l.plane       ? launch_plane<T>(sh, rt, pd.p + l.off, l.n, st, plane_variant, split)
+                           : l.stage >= 0 ? launch_xetd<T>(sh, rt, ed.p + l.off, l.n, l.stage, st)
+                                          : launch_xpass<T>(sh, rt, xd.p + l.off, l.n, st, x_variant, split);
             if (e != hipSuccess) return fail(PO_ERR_DEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
         }
         return PO_OK;
@@ -456,7 +629,8 @@ template <typename T> struct Program {
             }
             return PO_OK;
         };
-        const int nsp = (split_supported(sh) && !has_tab) ? 2 : 1;
+        // ETDRK4 programs have one x-pencil kernel and the unsplit layout
+        const int nsp = (split_supported(sh) && !has_tab && !etd) ? 2 : 1;
         int pin_p, pin_x, pin_s, rc;
         // PO_SPLIT=1 means "split-y wherever the grid has the kernels"
         if ((rc = pin("PO_SPLIT", 2, pin_s))) return rc;
@@ -466,7 +640,7 @@ template <typename T> struct Program {
         for (int sp = 0; sp < nsp; ++sp) {
             const long rows = sp ? sh.Ly / 2 : sh.Ly;
             const int npv = rows * sh.Lz >= 8192 ? 2 : 1;
-            const int nxv = xpass_variant(sh, 1, sp) == 1 ? 2 : 1;
+            const int nxv = (!etd && xpass_variant(sh, 1, sp) == 1) ? 2 : 1;
             if ((rc = pin("PO_PLANE_VARIANT", 2, pin_p)) || (rc = pin("PO_XPASS_VARIANT", 2, pin_x))) return rc;
             for (int p = 0; p < npv; ++p)
                 for (int x = 0; x < nxv; ++x)

@@ -178,6 +178,8 @@ AlgorithmMDEType Config::algo_mde_type() const {
 }
 std::string Config::get_algo_mde_type_string() const { return get_string("Algorithm_MDE", "algorithm"); }
 std::vector<double> Config::ds() const { return get_list("Algorithm_MDE", "ds"); }
+int Config::etdrk4_M() const { return get_int("Algorithm_MDE", "etdrk4_M", 0); }
+
 std::vector<unsigned> Config::Ms() const {
     std::vector<unsigned> out;
     for (double d : get_list("Algorithm_MDE", "Ms")) out.push_back((unsigned)std::lround(d));

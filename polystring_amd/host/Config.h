@@ -72,6 +72,7 @@ public:
     std::string get_algo_mde_type_string() const;
     std::vector<double> ds() const;
     std::vector<unsigned> Ms() const;
+    int etdrk4_M() const;  // src/Config.cc:1203-1205 (0 when absent)
     // [Algorithm_SCFT]
     AlgorithmSCFTType algo_scft_type() const;
     std::vector<double> lam() const;

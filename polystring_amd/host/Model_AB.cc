@@ -28,8 +28,11 @@ void Model_AB::init() {
         std::cerr << "Model_AB: this build runs the incompressible Anderson-mixing path only" << std::endl;
         std::exit(1);
     }
-    if (_cfg.algo_mde_type() == AlgorithmMDEType::ETDRK4) {
-        std::cerr << "Model_AB: [Algorithm_MDE] algorithm must be RQM4 or OS2" << std::endl;
+    // ETDRK4 runs the Cox-Matthews Etdrk4_PBC updater with M = 32 contour
+    // points whatever etdrk4_M holds (src/Model_AB.cc:747-762 use 32 unless
+    // etdrk4_M > 0)
+    if (_cfg.algo_mde_type() == AlgorithmMDEType::ETDRK4 && _cfg.etdrk4_M() > 0 && _cfg.etdrk4_M() != 32) {
+        std::cerr << "Model_AB: [Algorithm_MDE] etdrk4_M must be 32 (or <= 0) in this build" << std::endl;
         std::exit(1);
     }
     _uc = UnitCell(_cfg);
@@ -72,7 +75,9 @@ void Model_AB::init() {
     d.lamA = lam.size() > 0 ? lam[0] : 0.1;
     d.lamB = lam.size() > 1 ? lam[1] : 0.1;
     d.n_anderson = (int)_cfg.n_Anderson_mixing();
-    d.algo = _cfg.algo_mde_type() == AlgorithmMDEType::OS2 ? PO_OS2 : PO_RQM4;
+    d.algo = _cfg.algo_mde_type() == AlgorithmMDEType::OS2    ? PO_OS2
+             : _cfg.algo_mde_type() == AlgorithmMDEType::RQM4 ? PO_RQM4
+                                                                : PO_ETDRK4;
     d.contour = _cfg.algo_contour_integration_type() == AlgorithmContourType::TRAPEZOIDAL ? PO_QUAD_TRAP : PO_QUAD_AUTO;
     po_host_check(po_model_create(po_host_context(&_cfg), &d, wA.data(), wB.data(), &_m), "po_model_create");
 }

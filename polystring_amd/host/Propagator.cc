@@ -69,7 +69,12 @@ std::vector<double> Propagator::operator[](const int s) const {
 void Propagator::update(const Grid &w) { update(w, _updater); }
 
 void Propagator::update(const Grid &w, const Updater *up) {
-    const int algo = up && up->mde_algo() >= 0 ? up->mde_algo() : PO_RQM4;
+    const int algo = up && up->mde_algo() >= 0 ? up->mde_algo() : PO_ETDRK4;
+    if (const auto *e = dynamic_cast<const Etdrk4_PBC *>(up))
+        if (e->contour_points() != 32) {
+            std::cerr << "Etdrk4_PBC: only M = 32 contour points are built" << std::endl;
+            std::exit(1);
+        }
     ensure(algo);
     po_host_check(po_prop_solve(_h, w.data()), "propagator solve");
 }

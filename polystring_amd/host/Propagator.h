@@ -97,6 +97,25 @@ private:
     double _ds = 0.0;
 };
 
+// Etdrk4_PBC(uc, Lx, Ly, Lz, ds, M = 32) (include/Etdrk4_PBC.h:47-50): the
+// Cox-Matthews ETDRK4 updater.  Only M = 32 contour points are built.
+class Etdrk4_PBC : public Updater {
+public:
+    Etdrk4_PBC() {}
+    Etdrk4_PBC(const UnitCell &uc, const int Lx, const int Ly, const int Lz, const double ds, const int M = 32)
+        : _uc(uc), _ds(ds), _M(M) {
+        (void)Lx; (void)Ly; (void)Lz;
+    }
+    Etdrk4_PBC *clone() const override { return new Etdrk4_PBC(*this); }
+    int mde_algo() const override { return PO_ETDRK4; }
+    int contour_points() const { return _M; }
+
+private:
+    UnitCell _uc;
+    double _ds = 0.0;
+    int _M = 32;
+};
+
 class Simpson : public Updater {
 public:
     Simpson *clone() const override { return new Simpson; }
@@ -111,8 +130,8 @@ public:
 
 // Contour slices q(s), s = 0 .. len-1, on the GPU (include/Propagator.h:47-89).
 // Like the reference, the propagator does not own its Updater
-// (src/Propagator.cc:107-116); without one it defaults to RQM4 here (the
-// reference's default, ETDRK4, is not part of this build).
+// (src/Propagator.cc:107-116); without one it runs Etdrk4_PBC, the
+// reference's default updater.
 class Propagator : public Grid {
 public:
     Propagator(const std::string name, const Config &cfg, const int len, const double ds, Updater *up = nullptr);

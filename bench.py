@@ -34,13 +34,13 @@ HBM_PEAK_GBS = 8000.0
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "traffic_rqm4_step_128cubed.json")
 
 
-def run_form(precision, steps, warmup, adjoint, L=128, Ns=200):
+def run_form(precision, steps, warmup, adjoint, L=128, Ns=200, algo="RQM4"):
     import numpy as np
 
     from polystring_amd import capi, workloads
     wl = workloads.cfg2(L, Ns)
     ctx = capi.Context(0, precision)
-    m = capi.ModelAB(ctx, wl.wA, wl.wB, **wl.model_kwargs())
+    m = capi.ModelAB(ctx, wl.wA, wl.wB, algo=algo, **wl.model_kwargs())
     if adjoint:
         m.set_adjoint(True)
     # first update from the initial fields: the same-input precision probe.
@@ -72,7 +72,10 @@ def run_form(precision, steps, warmup, adjoint, L=128, Ns=200):
     p = 8 if precision == "fp64" else 4
     # dominant kernel: the propagator contour step (k_plane + k_xpass launches
     # of one lockstep step, which advances both chains by one contour step)
-    bytes_launch = 2 * workloads.algorithmic_bytes_per_step(wl.shape, p)
+    if algo == "ETDRK4":
+        bytes_launch = 2 * workloads.etdrk4_bytes_per_step(wl.shape, p)
+    else:
+        bytes_launch = 2 * workloads.algorithmic_bytes_per_step(wl.shape, p)
     if adjoint:
         bytes_launch += workloads.adjoint_bytes_per_lockstep_step(wl.shape, p)
     n_launch = prof["iters"] * prof["lockstep_steps"]
@@ -220,6 +223,13 @@ def main():
         print(f"[bench] string {json.dumps(extras['string_batched'])}", file=sys.stderr, flush=True)
         extras["ragged_scan"] = run_scan()
         print(f"[bench] scan {json.dumps(extras['ragged_scan'])}", file=sys.stderr, flush=True)
+        etd, _ = run_form("fp64", 10, 2, adjoint=False, algo="ETDRK4")
+        extras["etdrk4_fp64"] = {
+            "workload": "the same 128^3 Ns=200 system with the ETDRK4 updater (Etdrk4_PBC), FP64",
+            "contour_steps_per_s": etd["contour_steps_per_s"], "iters_per_s": etd["iters_per_s"],
+            "lockstep_step_us": etd["lockstep_step_us"], "achieved_GBs": etd["achieved_GBs"],
+            "bytes_per_launch": etd["bytes_per_launch"], "H": etd["H"]}
+        print(f"[bench] etdrk4 {json.dumps(extras['etdrk4_fp64'])}", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu_baseline else cpu_baseline()
 
     line = {

@@ -176,6 +176,18 @@ def algorithmic_bytes_per_step(shape, p):
     return (13 * M + 12 * Mh) * p
 
 
+def etdrk4_bytes_per_step(shape, p):
+    """(5 M + 64 Mh) p: the bytes the ETDRK4 dataflow moves per contour step.
+    Four stages, each an x-pencil pass and a yz-plane pass that read and write
+    the spectrum (8 Mh p) and read w (M p); the slice store (M p); the
+    carried spectra v, acc, t (11 complex reads / writes, 22 Mh p) and the
+    coefficient tables E, E2, Q, f1, f2, f3 (10 reads, 10 Mh p)."""
+    Lx, Ly, Lz = shape
+    M = Lx * Ly * Lz
+    Mh = Lx * Ly * (Lz // 2 + 1)
+    return (5 * M + 64 * Mh) * p
+
+
 def adjoint_bytes_per_lockstep_step(shape, p):
     """Extra bytes of the fused stress adjoint per lockstep step (two chains):
     each chain stores three full spectra per step in the first half of the

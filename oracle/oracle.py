@@ -55,6 +55,7 @@ def _load():
     lib.or_model_create.restype = ctypes.c_void_p
     lib.or_model_destroy.argtypes = [ctypes.c_void_p]
     lib.or_model_update.argtypes = [ctypes.c_void_p]
+    lib.or_model_set_unnormalised.argtypes = [ctypes.c_void_p, ctypes.c_int]
     for name in ("or_model_H", "or_model_Hw", "or_model_Hs", "or_model_residual", "or_model_Q"):
         getattr(lib, name).argtypes = [ctypes.c_void_p]
         getattr(lib, name).restype = ctypes.c_double
@@ -183,7 +184,9 @@ class Model:
     """Model_AB (Anderson, incompressible) on the oracle."""
 
     def __init__(self, wA0, wB0, *, dim, cell, Ms, chiN, lam=(0.1, 0.1), n_anderson=20,
-                 mde="RQM4", contour="SIMPSON"):
+                 mde="RQM4", contour="SIMPSON", unnormalised_phi=False):
+        """unnormalised_phi: upstream polyorder-master's phi with cc = 1 instead
+        of the fork's 1/Q (the normalisation its published H values used)."""
         wA0 = np.ascontiguousarray(wA0, dtype=np.float64)
         wB0 = np.ascontiguousarray(wB0, dtype=np.float64)
         self.shape = wA0.shape
@@ -196,6 +199,8 @@ class Model:
                                         _p(wA0), _p(wB0))
         if not self._h:
             raise ValueError("oracle model creation failed")
+        if unnormalised_phi:
+            lib().or_model_set_unnormalised(self._h, 1)
 
     def __del__(self):
         if getattr(self, "_h", None):
@@ -237,7 +242,7 @@ class EMModel(Model):
     """Model_AB with simple mixing (the string method's iteration) on the oracle."""
 
     def __init__(self, wA0, wB0, *, dim, cell, Ms, chiN, lam=(0.1, 0.1, 1.0), mirror=False,
-                 compressible=False, mde="RQM4", contour="SIMPSON"):
+                 compressible=False, mde="RQM4", contour="SIMPSON", unnormalised_phi=False):
         wA0 = np.ascontiguousarray(wA0, dtype=np.float64)
         wB0 = np.ascontiguousarray(wB0, dtype=np.float64)
         self.shape = wA0.shape
@@ -250,6 +255,8 @@ class EMModel(Model):
                                      _p(wA0), _p(wB0))
         if not self._h:
             raise ValueError("oracle model creation failed")
+        if unnormalised_phi:
+            lib().or_model_set_unnormalised(self._h, 1)
 
     def set_fields(self, wA, wB):
         lib().or_model_set_fields(self._h, _p(np.ascontiguousarray(wA, dtype=np.float64)),

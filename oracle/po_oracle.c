@@ -681,6 +681,12 @@ struct or_model {
     int em, mirror, compressible;
     double lamA, lamB, lamY;
     double *yita;
+    /* upstream polyorder-master normalisation: phi with cc = 1 instead of 1/Q
+     * (zip:polyorder-master/src/Model_AB.cc:75-79 leave set_cc commented out;
+     * Density's cc defaults to 1.0, zip:polyorder-master/src/Density.cc:11-48).
+     * The published free energies were computed with it; converged H is
+     * gauge invariant for symmetric schemes, not for ETDRK4. */
+    int unnormalised;
 };
 
 or_model *or_model_create(int dim, int Lx, int Ly, int Lz,
@@ -781,8 +787,8 @@ static void em_update(or_model *m) {
     memcpy(m->qAc, m->qBc + (long)(m->sB - 1) * M, sizeof(double) * M);
     solve_prop(m, m->qAc, m->sA, m->dsA, m->lapA, wA);
     m->Q = or_mean((int)M, m->qB + (long)(m->sB - 1) * M);
-    density(m, m->sA, m->qA, m->qAc, 1.0 / m->Q, m->dsA, m->phiA);
-    density(m, m->sB, m->qB, m->qBc, 1.0 / m->Q, m->dsB, m->phiB);
+    density(m, m->sA, m->qA, m->qAc, (m->unnormalised ? 1.0 : 1.0 / m->Q), m->dsA, m->phiA);
+    density(m, m->sB, m->qB, m->qBc, (m->unnormalised ? 1.0 : 1.0 / m->Q), m->dsB, m->phiB);
     if (m->compressible) {
         /* src/Model_AB.cc:205-206,225-226: eta = lamYita (phiA + phiB - 1) */
         for (long i = 0; i < M; i++) {
@@ -816,6 +822,8 @@ or_model *or_em_create(int dim, int Lx, int Ly, int Lz, int cs, double a, double
     return m;
 }
 
+void or_model_set_unnormalised(or_model *m, int on) { m->unnormalised = on; }
+
 void or_model_update(or_model *m) {
     if (m->em) { em_update(m); return; }
     long M = m->M;
@@ -829,8 +837,8 @@ void or_model_update(or_model *m) {
     solve_prop(m, m->qAc, m->sA, m->dsA, m->lapA, wA);
     /* :195-199 */
     m->Q = or_mean((int)M, m->qB + (long)(m->sB - 1) * M);
-    density(m, m->sA, m->qA, m->qAc, 1.0 / m->Q, m->dsA, m->phiA);
-    density(m, m->sB, m->qB, m->qBc, 1.0 / m->Q, m->dsB, m->phiB);
+    density(m, m->sA, m->qA, m->qAc, (m->unnormalised ? 1.0 : 1.0 / m->Q), m->dsA, m->phiA);
+    density(m, m->sB, m->qB, m->qBc, (m->unnormalised ? 1.0 : 1.0 / m->Q), m->dsB, m->phiB);
     /* :231-253, incompressible Anderson branch */
     double *wtA = m->W, *wtB = m->D;
     for (long i = 0; i < M; i++) {

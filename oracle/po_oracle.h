@@ -20,6 +20,12 @@
  *   or_rqm4_solve/os2    : single Fourier mode exp(-k^2 s); constant field
  *                          exp(-c s); 4th- vs 2nd-order convergence; adjoint
  *                          symmetry <q(s) qc(N-s)> = Q; published H values.
+ *   or_etdrk4_coeffs     : z -> 0 limits ds/2, ds/6 and the closed forms of
+ *                          the phi functions where well conditioned.
+ *   or_etdrk4_solve      : single Fourier mode, constant field, 4th-order
+ *                          gap to RQM4; with upstream's phi normalisation
+ *                          (or_model_set_unnormalised) the published LAM 1-D
+ *                          ETDRK4+Anderson and ETDRK4+EM H values.
  *   or_simpson/quad4/trap: closed form exp(fA)-1 (tests/test_quadrature4.cc).
  *   or_model_*           : published converged H (polyorder-master example
  *                          LAM 1D/2D/3D param.ini headers), <phiA>=fA.
@@ -114,6 +120,10 @@ or_model *or_model_create(int dim, int Lx, int Ly, int Lz,
 void or_model_destroy(or_model *m);
 /* One Model_AB::update() (src/Model_AB.cc:136-146,195-199,231-253). */
 void or_model_update(or_model *m);
+/* Upstream polyorder-master's phi normalisation cc = 1 instead of the
+ * fork's 1/Q (zip:polyorder-master/src/Model_AB.cc:75-79): the setting the
+ * published H values were computed with.  Test use only. */
+void or_model_set_unnormalised(or_model *m, int on);
 /* Model_AB::H, Hw, Hs (src/Model_AB.cc:263-300) and residual_error
  * (src/Model_AB.cc:307-315) evaluated after update(), as scft::relax does. */
 double or_model_H(const or_model *m);

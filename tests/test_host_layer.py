@@ -22,8 +22,8 @@ int main(int argc, char **argv) {
     Config cfg(argv[1]);
     printf("%u %u %u %u\n", cfg.dim(), cfg.Lx(), cfg.Ly(), cfg.Lz());
     printf("%.17g %.17g %.17g\n", cfg.a(), cfg.chiN().at(0), cfg.thresh_residual());
-    printf("%u %u %u %d %d\n", cfg.Ms()[0], cfg.Ms()[1], cfg.n_Anderson_mixing(),
-           (int)cfg.algo_mde_type(), (int)cfg.get_crystal_system_type());
+    printf("%u %u %u %d %d %d\n", cfg.Ms()[0], cfg.Ms()[1], cfg.n_Anderson_mixing(),
+           (int)cfg.algo_mde_type(), (int)cfg.get_crystal_system_type(), cfg.etdrk4_M());
     std::map<std::string, mat5::Var> v;
     std::string err;
     if (!mat5::read(argv[2], v, err)) { printf("ERR %s\n", err.c_str()); return 1; }
@@ -90,11 +90,17 @@ def test_config_and_mat5_roundtrip(tmp_path):
     lines = out.split("\n")
     assert lines[0].split() == ["1", "64", "1", "1"]
     assert [float(x) for x in lines[1].split()] == [4.0, 30.0, 1e-10]
-    assert lines[2].split() == ["51", "51", "5", "2", "0"]   # RQM4, LAMELLAR
+    assert lines[2].split() == ["51", "51", "5", "2", "0", "0"]   # RQM4, LAMELLAR, etdrk4_M absent
     assert [float(x) for x in lines[3].split()] == [w[0, 0, 0], w[1, 2, 1]]
     back = sio.loadmat(str(tmp_path / "out.mat"))
     assert np.array_equal(back["wA"], w)
     assert back["x"].item() == 2.5 and str(back["name"][0]) == "Cubic"
+    # ETDRK4 / ETDRK and unknown words select ETDRK4 (src/Config.cc:1099-1110)
+    for word, code in (("ETDRK4", "0"), ("etdrk", "0"), ("OS4", "2"), ("Pseudospectral", "1"), ("bogus", "0")):
+        ini.write_text(INI.replace("algorithm = RQM4", f"algorithm = {word}\netdrk4_M = 32"))
+        out = subprocess.check_output([str(exe), str(ini), str(tmp_path / "in.mat"), str(tmp_path / "o2.mat")],
+                                      text=True)
+        assert out.split("\n")[2].split()[3:] == [code, "0", "32"], word
 
 
 @pytest.mark.gpu
@@ -116,3 +122,20 @@ def test_driver_reproduces_published_lam1d(tmp_path):
     assert saved, os.listdir(tmp_path)
     m = sio.loadmat(str(tmp_path / sorted(saved)[0]))
     assert m["phiA"].shape == (64, 1, 1) and abs(m["phiA"].mean() - 0.5) < 1e-12
+
+
+@pytest.mark.gpu
+def test_driver_runs_etdrk4(tmp_path):
+    """param.ini with algorithm = ETDRK4: one update from the published field
+    gives H within 1e-6 of the published ETDRK4+Anderson value."""
+    f = np.load(os.path.join(GOLD, "lam1d_field.npz"))
+    sio.savemat(str(tmp_path / "field_in_64.mat"), {"wA": f["wA"], "wB": f["wB"]})
+    ini = INI.replace("algorithm = RQM4", "algorithm = ETDRK4\netdrk4_M =").replace("min_iter = 10", "min_iter = 0")
+    (tmp_path / "param.ini").write_text(ini.replace("max_iter = 2000", "max_iter = 1"))
+    print("\n[driver] running polyorder_amd with ETDRK4", flush=True)
+    out = subprocess.run([os.path.join(PKG, "polyorder_amd"), "param.ini"], cwd=tmp_path, capture_output=True,
+                         text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    done = dict(l.split(":\t", 1) for l in (tmp_path / "done.txt").read_text().splitlines() if ":\t" in l)
+    Hpub = json.load(open(os.path.join(GOLD, "known_answers.json")))["1D"]["ETDRK4+Anderson"]["H"]
+    assert abs(float(done["H"]) - Hpub) < 1e-6

@@ -439,6 +439,7 @@ template <typename T> struct Model : po_model {
         prog_adj.plane_variant = prog.plane_variant;
         prog_adj.x_variant = prog.x_variant;
         prog_adj.split = prog.split;
+        prog_adj.hold = prog.hold;
         fused_on = true;
         return PO_OK;
     }

@@ -131,9 +131,11 @@ bool xpass_supported(int Lx);
 // variant 0: default workgroup size; 1: 1024 threads (planes of >= 8192 reals)
 // split = 1: the split-y layout (half-plane kernels + radix-2 y stage in the
 // x-pencil kernel); plane and x-pencil launches of one program must agree.
+// hold = 1: FP64 half planes keep the second slice copy in registers (FP32
+// always does); ignored elsewhere.
 template <typename T>
 hipError_t launch_plane(const Shape &sh, const Roots<T> &rt, const PlaneDesc<T> *d_desc, int nlanes,
-                        hipStream_t st, int variant = 0, int split = 0);
+                        hipStream_t st, int variant = 0, int split = 0, int hold = 0);
 template <typename T>
 hipError_t launch_xpass(const Shape &sh, const Roots<T> &rt, const XDesc<T> *d_desc, int nlanes,
                         hipStream_t st, int xvariant = 0, int split = 0);

@@ -281,6 +281,12 @@ __device__ __forceinline__ void fft_wave(cx<T> *buf, int ps, int es, const cx<T>
 // radix at each end of a forward/inverse pair matches.
 enum { IO_LDS = 0, IO_REG = 1, IO_GLB = 2 };
 
+// LDS index of element e of pencil p: p * ps + e * es.  Kernels with a
+// swizzled LDS layout pass their own map with the same call signature.
+struct LinearMap {
+    __device__ __forceinline__ int operator()(int p, int e, int ps, int es) const { return p * ps + e * es; }
+};
+
 template <int N, bool INV, int S> struct StageRadix {
     static constexpr int value = INV ? Plan<N>::r[Plan<N>::n - 1 - S] : Plan<N>::r[S];
 };
@@ -293,9 +299,10 @@ template <int N, bool INV, int GP> struct WaveEnds {
     static constexpr int ITL = (GP * (N / RL) + 63) / 64;
 };
 
-template <typename T, int N, int R, int L, bool INV, int GP, bool PFAST, int IN, int OUT, int IT>
+template <typename T, int N, int R, int L, bool INV, int GP, bool PFAST, int IN, int OUT, int IT, class Map>
 __device__ __forceinline__ void wave_stage_io(cx<T> *lds, int ps, int es, const cx<T> *__restrict__ roots, int lane,
-                                              const cx<T> *gin, int gps, int ges, cx<T> *gout, cx<T> (&v)[IT][R]) {
+                                              const cx<T> *gin, int gps, int ges, cx<T> *gout, cx<T> (&v)[IT][R],
+                                              const Map &map) {
     constexpr int NB = N / R;
     constexpr int TOT = GP * NB;
     constexpr int TS = N / (L * R);
@@ -308,7 +315,7 @@ __device__ __forceinline__ void wave_stage_io(cx<T> *lds, int ps, int es, const 
             const int j = PFAST ? b / GP : b % NB;
             if constexpr (IN == IO_LDS) {
 #pragma unroll
-                for (int r = 0; r < R; ++r) v[it][r] = lds[p * ps + (j + r * NB) * es];
+                for (int r = 0; r < R; ++r) v[it][r] = lds[map(p, j + r * NB, ps, es)];
             } else if constexpr (IN == IO_GLB) {
 #pragma unroll
                 for (int r = 0; r < R; ++r) v[it][r] = gload(gin + (unsigned)(p * gps + (j + r * NB) * ges));
@@ -330,9 +337,8 @@ __device__ __forceinline__ void wave_stage_io(cx<T> *lds, int ps, int es, const 
                 const int p = PFAST ? b % GP : b / NB;
                 const int j = PFAST ? b / GP : b % NB;
                 const int k = j % L;
-                cx<T> *dst = lds + p * ps + ((j / L) * L * R + k) * es;
 #pragma unroll
-                for (int r = 0; r < R; ++r) dst[r * L * es] = v[it][r];
+                for (int r = 0; r < R; ++r) lds[map(p, (j / L) * L * R + k + r * L, ps, es)] = v[it][r];
             }
         }
         wave_sync();
@@ -360,31 +366,33 @@ struct WaveFFT {
     static constexpr int IT = (GP * (N / R) + 63) / 64;
     using E = WaveEnds<N, INV, GP>;
     using Next = WaveFFT<T, N, INV, GP, PFAST, IN, OUT, S + 1, L * R>;
+    template <class Map = LinearMap>
     static __device__ __forceinline__ void run(cx<T> *lds, int ps, int es, const cx<T> *roots, int lane,
                                                const cx<T> *gin, int gps, int ges, cx<T> *gout,
-                                               cx<T> (&vin)[E::ITF][E::RF], cx<T> (&vout)[E::ITL][E::RL]) {
+                                               cx<T> (&vin)[E::ITF][E::RF], cx<T> (&vout)[E::ITL][E::RL],
+                                               const Map &map = Map()) {
         constexpr bool first = S == 0, last = S == NS - 1;
         constexpr int in_mode = first ? IN : IO_LDS;
         constexpr int out_mode = last ? OUT : IO_LDS;
         if constexpr (first && IN == IO_REG) {
             static_assert(!(last && OUT == IO_REG) || NS == 1, "");
             wave_stage_io<T, N, R, L, INV, GP, PFAST, in_mode, out_mode, IT>(lds, ps, es, roots, lane, gin, gps, ges,
-                                                                            gout, vin);
+                                                                            gout, vin, map);
             if constexpr (last && OUT == IO_REG) {
 #pragma unroll
                 for (int i = 0; i < IT; ++i)
 #pragma unroll
                     for (int r = 0; r < R; ++r) vout[i][r] = vin[i][r];
             }
-            if constexpr (!last) Next::run(lds, ps, es, roots, lane, gin, gps, ges, gout, vin, vout);
+            if constexpr (!last) Next::run(lds, ps, es, roots, lane, gin, gps, ges, gout, vin, vout, map);
         } else if constexpr (last && OUT == IO_REG) {
             wave_stage_io<T, N, R, L, INV, GP, PFAST, in_mode, out_mode, IT>(lds, ps, es, roots, lane, gin, gps, ges,
-                                                                            gout, vout);
+                                                                            gout, vout, map);
         } else {
             cx<T> v[IT][R];
             wave_stage_io<T, N, R, L, INV, GP, PFAST, in_mode, out_mode, IT>(lds, ps, es, roots, lane, gin, gps, ges,
-                                                                            gout, v);
-            if constexpr (!last) Next::run(lds, ps, es, roots, lane, gin, gps, ges, gout, vin, vout);
+                                                                            gout, v, map);
+            if constexpr (!last) Next::run(lds, ps, es, roots, lane, gin, gps, ges, gout, vin, vout, map);
         }
     }
 };

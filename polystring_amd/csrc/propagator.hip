@@ -21,6 +21,37 @@
 
 namespace po {
 
+// Column padding of the k_plane3 LDS rows: column c of a row sits at
+// c + (c >> PADSH), one spare slot after every 2^PADSH columns, and rows start
+// TGT complex apart modulo the 64 four-byte banks (-1: CPW).  The pad spreads
+// the stride-R Stockham writes of the z transforms over the banks while
+// keeping every address of an unrolled butterfly a per-lane base plus an
+// immediate offset.  Entries were chosen by a bank model of the kernel's
+// LDS access sequence (64 banks of 4 bytes; 16-byte complex in groups of 16
+// lanes, 8-byte complex in groups of 32), within the LDS that keeps the
+// intended workgroups per CU; in that model they cut the LDS cycles of the
+// 128 x 128 and 64 x 128 planes from 1.9-2.6x to 1.45-1.9x the conflict-free
+// count.  Shapes without an entry keep unpadded rows.
+template <typename T, int LY, int LZ, int NT> struct PlanePad {
+    static constexpr int PADSH = 0, TGT = -1;
+};
+#define PO_PAD(TT, LY, LZ, NT, SH, TG)                                                                     \
+    template <> struct PlanePad<TT, LY, LZ, NT> { static constexpr int PADSH = SH, TGT = TG; };
+PO_PAD(double, 128, 128, 1024, 5, 4)
+PO_PAD(double, 128, 128, 512, 4, 8)
+PO_PAD(double, 64, 128, 1024, 5, 3)
+PO_PAD(double, 64, 128, 512, 4, 8)
+PO_PAD(double, 64, 64, 256, 3, 8)
+PO_PAD(double, 32, 64, 128, 4, 2)
+PO_PAD(double, 32, 32, 64, 4, 9)
+PO_PAD(float, 128, 128, 1024, 5, 5)
+PO_PAD(float, 128, 128, 512, 3, 8)
+PO_PAD(float, 64, 128, 1024, 5, 5)
+PO_PAD(float, 64, 128, 512, 3, 8)
+PO_PAD(float, 64, 64, 256, 5, 17)
+PO_PAD(float, 32, 64, 128, 5, 17)
+#undef PO_PAD
+
 template <typename T, int LY, int LZ, int NTV = 0> struct PlaneCfg {
     static constexpr int H = LZ / 2;
     static constexpr int HZ = LZ == 1 ? 1 : H + 1;
@@ -34,12 +65,15 @@ template <typename T, int LY, int LZ, int NTV = 0> struct PlaneCfg {
     // rows (z pencils) and regular columns (y pencils) owned per wave
     static constexpr int RPW = LZ > 1 ? LY / NW : 0;
     static constexpr int CPW = LZ > 1 ? H / NW : 0;
-    // LDS row stride in complex: consecutive rows start CPW complex apart
-    // modulo the 64 four-byte banks, so a wave reading CPW adjacent columns
-    // down 64 / CPW rows touches every bank equally often
+    static constexpr int PADSH = LZ > 1 ? PlanePad<T, LY, LZ, NT>::PADSH : 0;
+    static constexpr int WIDTH = HZ + (PADSH ? H >> PADSH : 0);  // LDS slots per row
+    // LDS row stride in complex: consecutive rows start TGT (default CPW)
+    // complex apart modulo the 64 four-byte banks, so a wave reading CPW
+    // adjacent columns down 64 / CPW rows touches every bank equally often
     static constexpr int CW = sizeof(cx<T>) / 4;
     static constexpr int BANKMOD = 64 / CW;
-    static constexpr int RS = LZ == 1 ? 1 : HZ + ((CPW - HZ % BANKMOD) % BANKMOD + BANKMOD) % BANKMOD;
+    static constexpr int TGT = PlanePad<T, LY, LZ, NT>::TGT < 0 ? CPW : PlanePad<T, LY, LZ, NT>::TGT;
+    static constexpr int RS = LZ == 1 ? 1 : WIDTH + ((TGT - WIDTH % BANKMOD) % BANKMOD + BANKMOD) % BANKMOD;
     static constexpr int NL = LY * RS;  // LDS complex per plane
     static_assert(LZ == 1 || (LY % NW == 0 && H % NW == 0), "plane shape must split evenly over waves");
 };
@@ -232,54 +266,59 @@ k_plane(const PlaneDesc<T> *__restrict__ descs, Roots<T> rt, T invM) {
     }
 }
 
-// R2C split on the RPW rows a wave owns (after the forward z transform).
-template <typename T, int H, int RPW, int RS>
-__device__ __forceinline__ void rows_r2c_split(cx<T> *rows, const cx<T> *__restrict__ rz, int lane) {
+// R2C split on the RPW rows a wave owns (after the forward z transform);
+// at(y, c) is the LDS index of column c of the wave's row y.
+template <typename T, int H, int RPW, class At>
+__device__ __forceinline__ void rows_r2c_split(cx<T> *buf, const At &at, const cx<T> *__restrict__ rz, int lane) {
     constexpr int NPAIR = H / 2 + 1;
     for (int t = lane; t < RPW * NPAIR; t += 64) {
         const int y = t / NPAIR, k = t % NPAIR;
-        cx<T> *row = rows + y * RS;
         if (k == 0) {
-            const cx<T> z0 = row[0];
-            row[0] = mk<T>(z0.x + z0.y, (T)0);
-            row[H] = mk<T>(z0.x - z0.y, (T)0);
+            const cx<T> z0 = buf[at(y, 0)];
+            buf[at(y, 0)] = mk<T>(z0.x + z0.y, (T)0);
+            buf[at(y, H)] = mk<T>(z0.x - z0.y, (T)0);
         } else {
             const int k2 = H - k;
-            const cx<T> a = row[k], b = row[k2];
+            const cx<T> a = buf[at(y, k)], b = buf[at(y, k2)];
             const T hf = (T)0.5;
             const cx<T> ea = scl<T>(add<T>(a, conj<T>(b)), hf);
             const cx<T> oa = rot<false, T>(scl<T>(sub<T>(a, conj<T>(b)), hf));
             const cx<T> eb = scl<T>(add<T>(b, conj<T>(a)), hf);
             const cx<T> ob = rot<false, T>(scl<T>(sub<T>(b, conj<T>(a)), hf));
-            row[k] = add<T>(ea, cmul<false, T>(oa, rz[k]));
-            row[k2] = add<T>(eb, cmul<false, T>(ob, rz[k2]));
+            buf[at(y, k)] = add<T>(ea, cmul<false, T>(oa, rz[k]));
+            buf[at(y, k2)] = add<T>(eb, cmul<false, T>(ob, rz[k2]));
         }
     }
 }
 
 // C2R pre-split on the RPW rows a wave owns (before the inverse z transform).
-template <typename T, int H, int RPW, int RS>
-__device__ __forceinline__ void rows_c2r_split(cx<T> *rows, const cx<T> *__restrict__ rz, int lane) {
+template <typename T, int H, int RPW, class At>
+__device__ __forceinline__ void rows_c2r_split(cx<T> *buf, const At &at, const cx<T> *__restrict__ rz, int lane) {
     constexpr int NPAIR = H / 2 + 1;
     for (int t = lane; t < RPW * NPAIR; t += 64) {
         const int y = t / NPAIR, k = t % NPAIR;
-        cx<T> *row = rows + y * RS;
         if (k == 0) {
-            const cx<T> x0 = row[0], xh = row[H];
+            const cx<T> x0 = buf[at(y, 0)], xh = buf[at(y, H)];
             const cx<T> e = add<T>(x0, conj<T>(xh));
             const cx<T> o = sub<T>(x0, conj<T>(xh));
-            row[0] = add<T>(e, rot<true, T>(o));
+            buf[at(y, 0)] = add<T>(e, rot<true, T>(o));
         } else {
             const int k2 = H - k;
-            const cx<T> a = row[k], b = row[k2];
+            const cx<T> a = buf[at(y, k)], b = buf[at(y, k2)];
             const cx<T> ea = add<T>(a, conj<T>(b));
             const cx<T> oa = cmul<true, T>(sub<T>(a, conj<T>(b)), rz[k]);
             const cx<T> eb = add<T>(b, conj<T>(a));
             const cx<T> ob = cmul<true, T>(sub<T>(b, conj<T>(a)), rz[k2]);
-            row[k] = add<T>(ea, rot<true, T>(oa));
-            row[k2] = add<T>(eb, rot<true, T>(ob));
+            buf[at(y, k)] = add<T>(ea, rot<true, T>(oa));
+            buf[at(y, k2)] = add<T>(eb, rot<true, T>(ob));
         }
     }
+}
+
+template <typename T, int LYW, int LZ, int NT>
+__device__ __forceinline__ int plane_loc(int y, int c) {
+    using C = PlaneCfg<T, LYW, LZ, NT>;
+    return y * C::RS + c + (C::PADSH ? (c >> C::PADSH) : 0);
 }
 
 // 3-D plane kernel with register hand-off.  Same contract as k_plane.  The
@@ -298,8 +337,17 @@ __device__ __forceinline__ void rows_c2r_split(cx<T> *rows, const cx<T> *__restr
 // Y[k'] = E + W^k' O, Y[k'+LY/2] = E - W^k' O, runs in the x-pencil kernel
 // k_xpass4, where both halves of a column meet.  Half planes take half the
 // LDS, so two workgroups share a CU and overlap their memory and LDS phases.
-template <typename T, int LY, int LZ, int NTV, int SPLIT = 0>
-__global__ void __launch_bounds__((PlaneCfg<T, (SPLIT ? LY / 2 : LY), LZ, NTV>::NT))
+// Half planes of up to 512 threads are meant to run two workgroups per CU:
+// their register budget is capped at that occupancy (except the FP64 hold
+// variant, which trades the second workgroup for the held slice copy).
+template <typename T, int LYW, int LZ, int NTV, int SPLIT, int HOLD> struct PlaneOcc {
+    static constexpr int NT = PlaneCfg<T, LYW, LZ, NTV>::NT;
+    static constexpr int value = (SPLIT && NT <= 512 && !(sizeof(T) == 8 && HOLD)) ? (2 * (NT / 64) + 3) / 4 : 1;
+};
+
+template <typename T, int LY, int LZ, int NTV, int SPLIT = 0, int HOLD = (sizeof(T) == 4)>
+__global__ void __launch_bounds__((PlaneCfg<T, (SPLIT ? LY / 2 : LY), LZ, NTV>::NT),
+                                  (PlaneOcc<T, (SPLIT ? LY / 2 : LY), LZ, NTV, SPLIT, HOLD>::value))
 k_plane3(const PlaneDesc<T> *__restrict__ descs, Roots<T> rt, T invM) {
     constexpr int LYW = SPLIT ? LY / 2 : LY;  // rows (and y transform length) of this workgroup
     using C = PlaneCfg<T, LYW, LZ, NTV>;
@@ -319,7 +367,13 @@ k_plane3(const PlaneDesc<T> *__restrict__ descs, Roots<T> rt, T invM) {
     const long co = (long)blockIdx.x * (LY * HZ) + (long)hh * (LYW * HZ);
     const cx<T> *ry = SPLIT ? rt.yh : rt.y;  // roots of order LYW
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    cx<T> *rows = buf + w * RPW * RS;
+    constexpr int NTA = C::NT;
+    // LDS maps: y transforms over this wave's columns, the Nyquist column,
+    // and z transforms / splits over this wave's rows
+    const auto colmap = [w](int p, int e, int, int) { return plane_loc<T, LYW, LZ, NTA>(e, w * CPW + p); };
+    const auto nyqmap = [](int, int e, int, int) { return plane_loc<T, LYW, LZ, NTA>(e, H); };
+    const auto rowmap = [w](int p, int e, int, int) { return plane_loc<T, LYW, LZ, NTA>(w * RPW + p, e); };
+    const auto rowat = [w](int y, int c) { return plane_loc<T, LYW, LZ, NTA>(w * RPW + y, c); };
 
     // pair pointer of row-layout element (it, r) inside a real plane array:
     // a block-uniform base plus a 32-bit offset, so addresses stay one VGPR
@@ -335,49 +389,51 @@ k_plane3(const PlaneDesc<T> *__restrict__ descs, Roots<T> rt, T invM) {
         __syncthreads();  // LDS free
         {
             cx<T> a[YI::ITF][YI::RF], b[YI::ITL][YI::RL];
-            WaveFFT<T, LYW, true, CPW, true, IO_GLB, IO_LDS>::run(buf + w * CPW, 1, RS, ry, lane, src + co + w * CPW,
-                                                                 1, HZ, nullptr, a, b);
+            WaveFFT<T, LYW, true, CPW, true, IO_GLB, IO_LDS>::run(buf, 1, RS, ry, lane, src + co + w * CPW, 1, HZ,
+                                                                 nullptr, a, b, colmap);
         }
         if (w == NW - 1) {
             cx<T> a[YI1::ITF][YI1::RF], b[YI1::ITL][YI1::RL];
-            WaveFFT<T, LYW, true, 1, true, IO_GLB, IO_LDS>::run(buf + H, 1, RS, ry, lane, src + co + H, 1, HZ,
-                                                               nullptr, a, b);
+            WaveFFT<T, LYW, true, 1, true, IO_GLB, IO_LDS>::run(buf, 1, RS, ry, lane, src + co + H, 1, HZ, nullptr, a,
+                                                               b, nyqmap);
         }
         __syncthreads();
-        rows_c2r_split<T, H, RPW, RS>(rows, rt.z, lane);
+        rows_c2r_split<T, H, RPW>(buf, rowat, rt.z, lane);
         wave_sync();
         {
             cx<T> a[ZI::ITF][ZI::RF];
-            WaveFFT<T, H, true, RPW, false, IO_LDS, IO_REG>::run(rows, RS, 1, rt.zh, lane, nullptr, 0, 0, nullptr, a,
-                                                                 v);
+            WaveFFT<T, H, true, RPW, false, IO_LDS, IO_REG>::run(buf, RS, 1, rt.zh, lane, nullptr, 0, 0, nullptr, a, v,
+                                                                 rowmap);
         }
     };
     auto forward_from_regs = [&](cx<T> (&v)[ITZ][RZ], cx<T> *dst) {
         __syncthreads();  // LDS free
         {
             cx<T> b[ZF::ITL][ZF::RL];
-            WaveFFT<T, H, false, RPW, false, IO_REG, IO_LDS>::run(rows, RS, 1, rt.zh, lane, nullptr, 0, 0, nullptr, v,
-                                                                  b);
+            WaveFFT<T, H, false, RPW, false, IO_REG, IO_LDS>::run(buf, RS, 1, rt.zh, lane, nullptr, 0, 0, nullptr, v,
+                                                                  b, rowmap);
         }
-        rows_r2c_split<T, H, RPW, RS>(rows, rt.z, lane);
+        rows_r2c_split<T, H, RPW>(buf, rowat, rt.z, lane);
         __syncthreads();
         {
             cx<T> a[YF::ITF][YF::RF], b[YF::ITL][YF::RL];
-            WaveFFT<T, LYW, false, CPW, true, IO_LDS, IO_GLB>::run(buf + w * CPW, 1, RS, ry, lane, nullptr, 1, HZ,
-                                                                  dst + co + w * CPW, a, b);
+            WaveFFT<T, LYW, false, CPW, true, IO_LDS, IO_GLB>::run(buf, 1, RS, ry, lane, nullptr, 1, HZ,
+                                                                  dst + co + w * CPW, a, b, colmap);
         }
         if (w == NW - 1) {
             cx<T> a[YF1::ITF][YF1::RF], b[YF1::ITL][YF1::RL];
-            WaveFFT<T, LYW, false, 1, true, IO_LDS, IO_GLB>::run(buf + H, 1, RS, ry, lane, nullptr, 1, HZ,
-                                                                dst + co + H, a, b);
+            WaveFFT<T, LYW, false, 1, true, IO_LDS, IO_GLB>::run(buf, 1, RS, ry, lane, nullptr, 1, HZ, dst + co + H,
+                                                                a, b, nyqmap);
         }
     };
 
     auto sq = [](cx<T> f) { return mk<T>(f.x * f.x, f.y * f.y); };
-    // FP32 has the registers to hold a second copy of the slice: q_f across
-    // the second inverse transform, then exp(-w ds/4) q' across the out0
-    // forward transform, instead of parking / re-reading them in HBM.
-    constexpr bool kHold = sizeof(T) == 4;
+    // HOLD: a second copy of the slice stays in registers -- q_f across the
+    // second inverse transform, then exp(-w ds/4) q' across the out0 forward
+    // transform -- instead of being parked in / re-read from HBM.  Default for
+    // FP32; an autotuned variant for FP64 half planes, where it costs the
+    // second workgroup per CU.
+    constexpr bool kHold = HOLD != 0;
     cx<T> q[ITZ][RZ];
     cx<T> qh[kHold ? ITZ : 1][kHold ? RZ : 1];
     bool stored = false, premul = false;  // qout written / out0 factor applied
@@ -1050,12 +1106,26 @@ bool split_supported(const Shape &sh) {
 
 template <typename T>
 hipError_t launch_plane(const Shape &sh, const Roots<T> &rt, const PlaneDesc<T> *d_desc, int nlanes,
-                        hipStream_t st, int variant, int split) {
+                        hipStream_t st, int variant, int split, int hold) {
     const T invM = (T)(1.0 / (double)sh.M);
 #define PO_CASE(a, b)                                                                                 \
     if (sh.Ly == a && sh.Lz == b) {                                                                   \
         if constexpr (a >= 32 && b > 1) {                                                             \
             if (split) {                                                                              \
+                if constexpr (sizeof(T) == 8) {                                                       \
+                    if (hold) {                                                                       \
+                        if constexpr ((a / 2) * b >= 8192) {                                          \
+                            if (variant == 1) {                                                       \
+                                hipLaunchKernelGGL((k_plane3<T, a, b, 1024, 1, 1>), dim3(sh.Lx, nlanes, 2), \
+                                                   dim3(1024), 0, st, d_desc, rt, invM);              \
+                                return hipGetLastError();                                             \
+                            }                                                                         \
+                        }                                                                             \
+                        hipLaunchKernelGGL((k_plane3<T, a, b, 0, 1, 1>), dim3(sh.Lx, nlanes, 2),      \
+                                           dim3(PlaneCfg<T, a / 2, b, 0>::NT), 0, st, d_desc, rt, invM); \
+                        return hipGetLastError();                                                     \
+                    }                                                                                 \
+                }                                                                                     \
                 if constexpr ((a / 2) * b >= 8192) {                                                  \
                     if (variant == 1) {                                                               \
                         hipLaunchKernelGGL((k_plane3<T, a, b, 1024, 1>), dim3(sh.Lx, nlanes, 2), dim3(1024), 0, st, \
@@ -1246,9 +1316,9 @@ template hipError_t launch_etd_gather<double>(const Shape &, const double *, dou
 template hipError_t launch_etd_gather<float>(const Shape &, const float *, float *, hipStream_t);
 
 template hipError_t launch_plane<double>(const Shape &, const Roots<double> &, const PlaneDesc<double> *, int,
-                                         hipStream_t, int, int);
+                                         hipStream_t, int, int, int);
 template hipError_t launch_plane<float>(const Shape &, const Roots<float> &, const PlaneDesc<float> *, int,
-                                        hipStream_t, int, int);
+                                        hipStream_t, int, int, int);
 template hipError_t launch_xpass<double>(const Shape &, const Roots<double> &, const XDesc<double> *, int,
                                          hipStream_t, int, int);
 template hipError_t launch_xpass<float>(const Shape &, const Roots<float> &, const XDesc<float> *, int,

@@ -604,7 +604,7 @@ template <typename T> struct Program {
 
     int run(const Shape &sh, const Roots<T> &rt, hipStream_t st) const {
         for (const auto &l : seq) {
-            hipError_t e = l.plane       ? launch_plane<T>(sh, rt, pd.p + l.off, l.n, st, plane_variant, split)
+            hipError_t e = l.plane       ? launch_plane<T>(sh, rt, pd.p + l.off, l.n, st, plane_variant, split, hold)
                            : l.stage >= 0 ? launch_xetd<T>(sh, rt, ed.p + l.off, l.n, l.stage, st)
                                           : launch_xpass<T>(sh, rt, xd.p + l.off, l.n, st, x_variant, split);
             if (e != hipSuccess) return fail(PO_ERR_DEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
@@ -613,14 +613,17 @@ template <typename T> struct Program {
     }
 
     // Autotuner: times the whole program once per launch-shape combination
-    // -- the yz-plane kernel's workgroup size (512 or 1024 threads) and the
-    // x-pencil tile width (16 or 32 columns) -- on the live buffers and keeps
-    // the fastest.  Every slice and spectrum the program writes is rewritten
-    // by the next real run, so tuning leaves no trace in the results.  All
-    // variants compute the same arithmetic in the same order.
+    // -- spectrum layout (full or split-y), the yz-plane kernel's workgroup
+    // size (512 or 1024 threads), the x-pencil tile width (16 or 32 columns)
+    // and, for FP64 half planes, whether the second slice copy is held in
+    // registers -- on the live buffers and keeps the fastest.  Every slice
+    // and spectrum the program writes is rewritten by the next real run, so
+    // tuning leaves no trace in the results.  All variants compute the same
+    // arithmetic in the same order.
     int tune(const Shape &sh, const Roots<T> &rt, hipStream_t st) {
-        // PO_PLANE_VARIANT / PO_XPASS_VARIANT / PO_SPLIT = 0|1 pin the choices
-        // (reproducible launch shapes) wherever the grid has that kernel
+        // PO_PLANE_VARIANT / PO_XPASS_VARIANT / PO_SPLIT / PO_PLANE_HOLD = 0|1
+        // pin the choices (reproducible launch shapes) wherever the grid has
+        // that kernel
         auto pin = [](const char *name, int n, int &out) -> int {
             out = -1;
             if (const char *env = std::getenv(name)) {
@@ -635,27 +638,34 @@ template <typename T> struct Program {
         // PO_SPLIT=1 means "split-y wherever the grid has the kernels"
         if ((rc = pin("PO_SPLIT", 2, pin_s))) return rc;
         if (pin_s >= nsp) pin_s = 0;
-        struct Cand { int p, x, s; };
+        struct Cand { int p, x, s, h; };
         std::vector<Cand> cand;
+        int pin_h;
+        if ((rc = pin("PO_PLANE_VARIANT", 2, pin_p)) || (rc = pin("PO_XPASS_VARIANT", 2, pin_x)) ||
+            (rc = pin("PO_PLANE_HOLD", 2, pin_h)))
+            return rc;
         for (int sp = 0; sp < nsp; ++sp) {
             const long rows = sp ? sh.Ly / 2 : sh.Ly;
             const int npv = rows * sh.Lz >= 8192 ? 2 : 1;
             const int nxv = (!etd && xpass_variant(sh, 1, sp) == 1) ? 2 : 1;
-            if ((rc = pin("PO_PLANE_VARIANT", 2, pin_p)) || (rc = pin("PO_XPASS_VARIANT", 2, pin_x))) return rc;
+            const int nh = (sizeof(T) == 8 && sp) ? 2 : 1;
             for (int p = 0; p < npv; ++p)
                 for (int x = 0; x < nxv; ++x)
-                    if ((pin_p < 0 || p == pin_p) && (pin_x < 0 || x == pin_x) && (pin_s < 0 || sp == pin_s))
-                        cand.push_back({p, x, sp});
+                    for (int h = 0; h < nh; ++h)
+                        if ((pin_p < 0 || p == pin_p) && (pin_x < 0 || x == pin_x) && (pin_s < 0 || sp == pin_s) &&
+                            (pin_h < 0 || h == pin_h))
+                            cand.push_back({p, x, sp, h});
         }
         if (cand.empty()) {
             // a pin this grid has no kernel for falls back to the unpinned choice
             for (int sp = 0; sp < nsp; ++sp)
-                if (pin_s < 0 || sp == pin_s) cand.push_back({0, 0, sp});
+                if (pin_s < 0 || sp == pin_s) cand.push_back({0, 0, sp, 0});
         }
         if (cand.size() == 1) {
             plane_variant = cand[0].p;
             x_variant = cand[0].x;
             split = cand[0].s;
+            hold = cand[0].h;
             return PO_OK;
         }
         hipEvent_t a, b;
@@ -667,6 +677,7 @@ template <typename T> struct Program {
             plane_variant = c.p;
             x_variant = c.x;
             split = c.s;
+            hold = c.h;
             rc = run(sh, rt, st);  // warm-up
             if (rc) return rc;
             PO_HIP(hipEventRecord(a, st));
@@ -676,7 +687,6 @@ template <typename T> struct Program {
             PO_HIP(hipEventSynchronize(b));
             float ms = 0.f;
             PO_HIP(hipEventElapsedTime(&ms, a, b));
-            tuned_ms[c.s * 4 + c.p * 2 + c.x] = ms / 2;
             if (ms < best) { best = ms; pick = c; }
         }
         (void)hipEventDestroy(a);
@@ -684,6 +694,7 @@ template <typename T> struct Program {
         plane_variant = pick.p;
         x_variant = pick.x;
         split = pick.s;
+        hold = pick.h;
         return PO_OK;
     }
 
@@ -722,7 +733,7 @@ template <typename T> struct Program {
     int x_variant = 0;
     int split = 0;         // split-y layout of the spectra
     bool has_tab = false;  // a full k table (general-cell OS2): no split-y kernel for it
-    float tuned_ms[8] = {};  // [split * 4 + plane variant * 2 + x variant]
+    int hold = 0;          // FP64 half planes hold the second slice copy in registers
     int use_graph = -1;
     hipGraphExec_t exec = nullptr;
 };

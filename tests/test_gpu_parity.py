@@ -203,7 +203,8 @@ def test_graph_replay_matches_direct_launches(ctx64, monkeypatch):
 
 def test_launch_variants_agree(ctx64, monkeypatch):
     """Every launch-shape variant the autotuner can pick (full or split-y
-    planes, 512 / 1024-thread planes, 16 / 32-column x-pencil tiles) runs the
+    planes, 512 / 1024-thread planes, 16 / 32-column x-pencil tiles, FP64 half
+    planes holding the second slice copy in registers or in HBM) runs the
     same transforms (the split-y form factors the y transform once more) in the
     same order; only the compiler's FMA contraction may differ between the
     instantiations, so pinned variants agree to rounding (1e-13 here), and
@@ -217,23 +218,24 @@ def test_launch_variants_agree(ctx64, monkeypatch):
     ref = capi.ModelAB(ctx64, wA, wB, **kw)
     g_ref, _ = ref.stress()
     out = {}
-    for sp in (0, 1):
-        for pv in (0, 1):
-            for xv in (0, 1):
-                monkeypatch.setenv("PO_SPLIT", str(sp))
-                monkeypatch.setenv("PO_PLANE_VARIANT", str(pv))
-                monkeypatch.setenv("PO_XPASS_VARIANT", str(xv))
-                m = capi.ModelAB(ctx64, wA, wB, **kw)  # pins without a kernel fall back
-                m.set_adjoint(True)
-                m.update(2)
-                out[(sp, pv, xv)] = (m.observe()["H"], m.get("wA"))
-                m2 = capi.ModelAB(ctx64, wA, wB, **kw)
-                m2.set_adjoint(True)
-                m2.update(1)
-                g = m2.last_stress()
-                print(f"  split {sp} plane {pv} x-pencil {xv}: H {out[(sp, pv, xv)][0]!r} stress {g}", flush=True)
-                assert np.abs(g - g_ref).max() <= 1e-12 * np.abs(g_ref).max()
-    assert (1, 0, 0) in out and (0, 1, 1) in out
-    H0, w0 = out[(0, 0, 0)]
+    combos = [(sp, pv, xv, 0) for sp in (0, 1) for pv in (0, 1) for xv in (0, 1)] + [(1, 0, 0, 1), (1, 1, 1, 1)]
+    for sp, pv, xv, hd in combos:
+        monkeypatch.setenv("PO_SPLIT", str(sp))
+        monkeypatch.setenv("PO_PLANE_VARIANT", str(pv))
+        monkeypatch.setenv("PO_XPASS_VARIANT", str(xv))
+        monkeypatch.setenv("PO_PLANE_HOLD", str(hd))
+        m = capi.ModelAB(ctx64, wA, wB, **kw)  # pins without a kernel fall back
+        m.set_adjoint(True)
+        m.update(2)
+        out[(sp, pv, xv, hd)] = (m.observe()["H"], m.get("wA"))
+        m2 = capi.ModelAB(ctx64, wA, wB, **kw)
+        m2.set_adjoint(True)
+        m2.update(1)
+        g = m2.last_stress()
+        print(f"  split {sp} plane {pv} x-pencil {xv} hold {hd}: H {out[(sp, pv, xv, hd)][0]!r} stress {g}",
+              flush=True)
+        assert np.abs(g - g_ref).max() <= 1e-12 * np.abs(g_ref).max()
+    assert (1, 0, 0, 1) in out and (0, 1, 1, 0) in out
+    H0, w0 = out[(0, 0, 0, 0)]
     for key, (H, w) in out.items():
         assert abs(H - H0) <= 1e-13 * abs(H0) and relerr(w, w0) <= 1e-13, key

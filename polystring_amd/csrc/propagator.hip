@@ -668,6 +668,40 @@ k_xdot(const DotDesc<T> *__restrict__ desc, Roots<T> rt, int ncol, int hz, int n
 // W = 0: 16-column tiles; W = 1: 32-column tiles with the same threads, so
 // each wave reads twice as many adjacent columns per row (64 B FP32, 128 B
 // FP64 segments) and half as many workgroups are launched.
+// LDS rows of the register hand-off x-pencil kernels: row x (an x position)
+// starts at x * RS + (x >> SH) * PADX.  The extra PADX slots after every 2^SH
+// rows move the rows of a Stockham stride-R write, which are 2^SH or more
+// apart, into distinct bank groups; addresses of an unrolled butterfly stay
+// a per-lane base plus an immediate.  Entries were chosen, like the plane
+// padding, by a bank model of the kernels' LDS access sequence, which puts
+// the 128-point passes at 1.1x the conflict-free LDS cycles (from 1.75x FP64
+// and 2.4x FP32).  Without an entry: SH = 0 and the plain stride.
+template <typename T, int LX, int W> struct X3Pad {
+    static constexpr int RS = -1, SH = 0, PADX = 0;
+};
+#define PO_XPAD(TT, LX, W, RSV, SHV, PV)                                                                   \
+    template <> struct X3Pad<TT, LX, W> { static constexpr int RS = RSV, SH = SHV, PADX = PV; };
+PO_XPAD(double, 48, 0, 24, 2, 8)
+PO_XPAD(double, 64, 0, 24, 2, 4)
+PO_XPAD(double, 96, 0, 28, 3, 4)
+PO_XPAD(double, 96, 1, 40, 3, 8)
+PO_XPAD(double, 128, 0, 20, 2, 2)
+PO_XPAD(double, 128, 1, 40, 2, 4)
+PO_XPAD(double, 256, 0, 18, 3, 2)
+PO_XPAD(double, 256, 1, 36, 2, 2)
+PO_XPAD(float, 32, 0, 16, 2, 8)
+PO_XPAD(float, 48, 0, 24, 2, 8)
+PO_XPAD(float, 48, 1, 48, 2, 16)
+PO_XPAD(float, 64, 0, 24, 2, 4)
+PO_XPAD(float, 64, 1, 48, 2, 8)
+PO_XPAD(float, 96, 0, 20, 3, 4)
+PO_XPAD(float, 96, 1, 56, 3, 8)
+PO_XPAD(float, 128, 0, 20, 3, 4)
+PO_XPAD(float, 128, 1, 40, 2, 4)
+PO_XPAD(float, 256, 0, 18, 4, 2)
+PO_XPAD(float, 256, 1, 36, 3, 4)
+#undef PO_XPAD
+
 template <typename T, int LX, int W = 0> struct X3Cfg {
     static constexpr int C = W ? 32 : 16;
     // LX*16/8 threads (one radix-8 butterfly each for a 16-column tile)
@@ -679,9 +713,15 @@ template <typename T, int LX, int W = 0> struct X3Cfg {
     // LDS row stride: a wave reads its CPW adjacent columns down 64 / CPW
     // rows; with RS an odd multiple of CPW the rows of one LDS cycle start in
     // distinct bank groups.  C / CPW = NW is a power of two, so C + CPW
-    // (or C itself for one wave) is the smallest such stride.
-    static constexpr int RS = NW == 1 ? C : C + CPW;
+    // (or C itself for one wave) is the smallest such stride.  X3Pad may
+    // override it and add row padding.
+    static constexpr int RS = X3Pad<T, LX, W>::RS > 0 ? X3Pad<T, LX, W>::RS : (NW == 1 ? C : C + CPW);
+    static constexpr int SH = X3Pad<T, LX, W>::SH, PADX = X3Pad<T, LX, W>::PADX;
+    static constexpr int NL = LX * RS + (SH ? (LX >> SH) * PADX : 0);  // LDS complex per workgroup
     static_assert(C % NW == 0, "tile columns must split evenly over waves");
+    static_assert(RS >= C, "row stride below the tile width");
+    // LDS index of tile column c of row x
+    static __device__ __forceinline__ int loc(int x, int c) { return x * RS + (SH ? (x >> SH) * PADX : 0) + c; }
 };
 
 template <typename T, int LX, int W>
@@ -693,16 +733,17 @@ k_xpass3(const XDesc<T> *__restrict__ descs, Roots<T> rt, int ncol, int hz) {
     using I = WaveEnds<LX, true, CPW>;
     static_assert(F::RL == I::RF && F::ITL == I::ITF, "x transform ends must match");
     constexpr int RX = F::RL, ITX = F::ITL, NBX = LX / RX, TOTX = CPW * NBX;
-    __shared__ cx<T> buf[LX * RS];
+    __shared__ cx<T> buf[X::NL];
     const XDesc<T> d = descs[blockIdx.y];
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int cw = blockIdx.x * X::C + w * CPW;  // first column of this wave
-    cx<T> *lds = buf + w * CPW;
+    const auto xmap = [w](int p, int e, int, int) { return X::loc(e, w * CPW + p); };
+    cx<T> *lds = buf;
     cx<T> v[ITX][RX];
     {
         cx<T> a[F::ITF][F::RF];
         WaveFFT<T, LX, false, CPW, true, IO_GLB, IO_REG>::run(lds, 1, RS, rt.x, lane, d.S + cw, 1, ncol, nullptr, a,
-                                                              v);
+                                                              v, xmap);
     }
     // full 3-D spectrum of this wave's columns, in register order (stored
     // for, or read from, the partner chain of the adjoint stress)
@@ -742,7 +783,7 @@ k_xpass3(const XDesc<T> *__restrict__ descs, Roots<T> rt, int ncol, int hz) {
     {
         cx<T> b[I::ITL][I::RL];
         WaveFFT<T, LX, true, CPW, true, IO_REG, IO_GLB>::run(lds, 1, RS, rt.x, lane, nullptr, 1, ncol,
-                                                             (d.Sout ? d.Sout : d.S) + cw, v, b);
+                                                             (d.Sout ? d.Sout : d.S) + cw, v, b, xmap);
     }
     if (d.Sdot) {
         // workgroup sum of the dot, through the (now free) transform buffer,
@@ -780,22 +821,23 @@ k_xpass4(const XDesc<T> *__restrict__ descs, Roots<T> rt, int ncol, int hz, int 
     using I = WaveEnds<LX, true, CPW>;
     static_assert(F::RL == I::RF && F::ITL == I::ITF, "x transform ends must match");
     constexpr int RX = F::RL, ITX = F::ITL, NBX = LX / RX, TOTX = CPW * NBX;
-    __shared__ cx<T> buf[LX * RS];
+    __shared__ cx<T> buf[X::NL];
     const XDesc<T> d = descs[blockIdx.y];
     const int half = ncol >> 1;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int cw = blockIdx.x * X::C + w * CPW;  // first half-column of this wave
-    cx<T> *lds = buf + w * CPW;
+    const auto xmap = [w](int p, int e, int, int) { return X::loc(e, w * CPW + p); };
+    cx<T> *lds = buf;
     cx<T> ve[ITX][RX], vo[ITX][RX];
     {
         cx<T> a[F::ITF][F::RF];
         WaveFFT<T, LX, false, CPW, true, IO_GLB, IO_REG>::run(lds, 1, RS, rt.x, lane, d.S + cw, 1, ncol, nullptr, a,
-                                                              ve);
+                                                              ve, xmap);
     }
     {
         cx<T> a[F::ITF][F::RF];
         WaveFFT<T, LX, false, CPW, true, IO_GLB, IO_REG>::run(lds, 1, RS, rt.x, lane, d.S + half + cw, 1, ncol,
-                                                              nullptr, a, vo);
+                                                              nullptr, a, vo, xmap);
     }
     // stored spectra: the lo set (ky = k') then the hi set (ky = k' + Ly/2) of this wave
     const unsigned sbase = (blockIdx.x * X::NW + w) * (unsigned)(2 * ITX * RX * 64);
@@ -840,12 +882,12 @@ k_xpass4(const XDesc<T> *__restrict__ descs, Roots<T> rt, int ncol, int hz, int 
     {
         cx<T> bb[I::ITL][I::RL];
         WaveFFT<T, LX, true, CPW, true, IO_REG, IO_GLB>::run(lds, 1, RS, rt.x, lane, nullptr, 1, ncol,
-                                                             (d.Sout ? d.Sout : d.S) + cw, ve, bb);
+                                                             (d.Sout ? d.Sout : d.S) + cw, ve, bb, xmap);
     }
     {
         cx<T> bb[I::ITL][I::RL];
         WaveFFT<T, LX, true, CPW, true, IO_REG, IO_GLB>::run(lds, 1, RS, rt.x, lane, nullptr, 1, ncol,
-                                                             (d.Sout ? d.Sout : d.S) + half + cw, vo, bb);
+                                                             (d.Sout ? d.Sout : d.S) + half + cw, vo, bb, xmap);
     }
     if (d.Sdot) {
         double *red = reinterpret_cast<double *>(buf);
@@ -874,24 +916,25 @@ k_xdot3(const DotDesc<T> *__restrict__ desc, Roots<T> rt, int ncol, int hz, int 
     constexpr int CPW = X::CPW, RS = X::RS, NT = X::NT;
     using F = WaveEnds<LX, false, CPW>;
     constexpr int RX = F::RL, ITX = F::ITL, NBX = LX / RX, TOTX = CPW * NBX;
-    __shared__ cx<T> buf[LX * RS];
+    __shared__ cx<T> buf[X::NL];
     __shared__ double red[3][NT / 64];
     const DotDesc<T> d = desc[blockIdx.y];
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int cw = blockIdx.x * X::C + w * CPW;
-    cx<T> *lds = buf + w * CPW;
+    const auto xmap = [w](int p, int e, int, int) { return X::loc(e, w * CPW + p); };
+    cx<T> *lds = buf;
     double acc[3] = {0.0, 0.0, 0.0};
     for (int p = 0; p < d.npair; ++p) {
         cx<T> va[ITX][RX], vb[ITX][RX];
         {
             cx<T> t[F::ITF][F::RF];
             WaveFFT<T, LX, false, CPW, true, IO_GLB, IO_REG>::run(lds, 1, RS, rt.x, lane, d.a[p] + cw, 1, ncol,
-                                                                  nullptr, t, va);
+                                                                  nullptr, t, va, xmap);
         }
         {
             cx<T> t[F::ITF][F::RF];
             WaveFFT<T, LX, false, CPW, true, IO_GLB, IO_REG>::run(lds, 1, RS, rt.x, lane, d.b[p] + cw, 1, ncol,
-                                                                  nullptr, t, vb);
+                                                                  nullptr, t, vb, xmap);
         }
 #pragma unroll
         for (int it = 0; it < ITX; ++it) {
@@ -974,18 +1017,19 @@ k_xetd3(const EtdDesc<T> *__restrict__ descs, Roots<T> rt, int ncol, int stage) 
     static_assert(F::RL == I::RF && F::ITL == I::ITF, "x transform ends must match");
     constexpr int RX = F::RL, ITX = F::ITL, NBX = LX / RX, TOTX = CPW * NBX;
     (void)NBX;
-    __shared__ cx<T> buf[LX * RS];
+    __shared__ cx<T> buf[X::NL];
     const EtdDesc<T> d = descs[blockIdx.y];
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int cw = blockIdx.x * X::C + w * CPW;
-    cx<T> *lds = buf + w * CPW;
+    const auto xmap = [w](int p, int e, int, int) { return X::loc(e, w * CPW + p); };
+    cx<T> *lds = buf;
     const unsigned sbase = (blockIdx.x * X::NW + w) * (unsigned)(ITX * RX * 64);
     cx<T> v[ITX][RX];
     if (d.Sinit) {
         {
             cx<T> a[F::ITF][F::RF];
             WaveFFT<T, LX, false, CPW, true, IO_GLB, IO_REG>::run(lds, 1, RS, rt.x, lane, d.Sinit + cw, 1, ncol,
-                                                                  nullptr, a, v);
+                                                                  nullptr, a, v, xmap);
         }
 #pragma unroll
         for (int it = 0; it < ITX; ++it)
@@ -996,7 +1040,7 @@ k_xetd3(const EtdDesc<T> *__restrict__ descs, Roots<T> rt, int ncol, int stage) 
     {
         cx<T> a[F::ITF][F::RF];
         WaveFFT<T, LX, false, CPW, true, IO_GLB, IO_REG>::run(lds, 1, RS, rt.x, lane, d.S + cw, 1, ncol, nullptr, a,
-                                                              v);
+                                                              v, xmap);
     }
 #pragma unroll
     for (int it = 0; it < ITX; ++it)
@@ -1006,7 +1050,8 @@ k_xetd3(const EtdDesc<T> *__restrict__ descs, Roots<T> rt, int ncol, int stage) 
                 v[it][r] = etd_point<T>(stage, d, sbase + (unsigned)((it * RX + r) * 64 + lane), v[it][r]);
     {
         cx<T> b[I::ITL][I::RL];
-        WaveFFT<T, LX, true, CPW, true, IO_REG, IO_GLB>::run(lds, 1, RS, rt.x, lane, nullptr, 1, ncol, d.S + cw, v, b);
+        WaveFFT<T, LX, true, CPW, true, IO_REG, IO_GLB>::run(lds, 1, RS, rt.x, lane, nullptr, 1, ncol, d.S + cw, v, b,
+                                                             xmap);
     }
 }
 

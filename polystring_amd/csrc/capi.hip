@@ -330,7 +330,18 @@ template <typename T> struct Model : po_model {
         const long M = sh.M;
         for (int it = 0; it < iters; ++it) {
             PO_HIP(mark());
-            int rc = (fused_on ? prog_adj : prog).run_graph(sh, rt, st);
+            Program<T> &pg = fused_on ? prog_adj : prog;
+            int rc;
+            if (use_dual()) {
+                PO_HIP(hipEventRecord(dev[0], st));
+                PO_HIP(hipStreamWaitEvent(st2, dev[0], 0));
+                rc = pg.run_dual(sh, rt, st, st2, dev[1], dev[2]);
+                if (rc) return rc;
+                PO_HIP(hipEventRecord(dev[3], st2));
+                PO_HIP(hipStreamWaitEvent(st, dev[3], 0));
+            } else {
+                rc = pg.run_graph(sh, rt, st);
+            }
             if (rc) return rc;
             if (fused_on) {
                 PO_HIP(launch_sum_rows(fusedPart.p,
@@ -447,6 +458,35 @@ template <typename T> struct Model : po_model {
     void set_graph(bool on) override {
         prog.use_graph = on ? 1 : 0;
         prog_adj.use_graph = on ? 1 : 0;
+    }
+
+    // The two chains on two streams (Program::run_dual), the default for
+    // direct (non-graph) launches; PO_DUAL=0 keeps both lanes in each launch
+    // on one stream.  The second stream and the fork / mid / join events are
+    // created on first use and live with the model.
+    int dual = -1;
+    hipStream_t st2 = nullptr;
+    hipEvent_t dev[4] = {};
+    bool use_dual() {
+        if (dual < 0) {
+            const char *env = std::getenv("PO_DUAL");
+            dual = (env && env[0] == '0') ? 0 : 1;
+            if (dual) {
+                bool ok = hipStreamCreateWithFlags(&st2, hipStreamNonBlocking) == hipSuccess;
+                for (auto &e : dev) ok = ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
+                if (!ok) dual = 0;
+            }
+        }
+        Program<T> &pg = fused_on ? prog_adj : prog;
+        return dual == 1 && !pg.graph_on() && lanes_.size() == 2;
+    }
+    ~Model() override {
+        if (st2) {
+            (void)hipStreamSynchronize(st2);
+            (void)hipStreamDestroy(st2);
+        }
+        for (auto e : dev)
+            if (e) (void)hipEventDestroy(e);
     }
 
     int last_stress(double *dHdl) override {

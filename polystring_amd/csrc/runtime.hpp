@@ -382,6 +382,9 @@ template <typename T> struct Program {
     std::vector<Launch> seq;
     int ndot = 0;  // dot descriptors of a fused adjoint program
     bool etd = false;  // an ETDRK4 program (unsplit layout, 16-column x tiles)
+    // first launch of the second half of the contour in a fused adjoint
+    // program: its x-passes read the partner lane's first-half spectra
+    int mid_launch = -1;
 
     int build(int algo, const std::vector<Lane<T>> &lanes, hipStream_t st, const AdjointArena<T> *adj = nullptr) {
         if (algo == PO_ETDRK4) {
@@ -389,6 +392,7 @@ template <typename T> struct Program {
             return build_etd(lanes, st);
         }
         etd = false;
+        mid_launch = -1;
         const int L = (int)lanes.size();
         int nsteps = -1;
         for (const auto &ln : lanes) {
@@ -447,6 +451,7 @@ template <typename T> struct Program {
             push_p(v);
         }
         for (int t = 0; t < nsteps; ++t) {
+            if (adj && t == nsteps / 2) mid_launch = (int)seq.size();
             std::vector<XDesc<T>> x2, x1;
             std::vector<PlaneDesc<T>> mid, fin;
             for (int l = 0; l < L; ++l) {
@@ -602,11 +607,41 @@ template <typename T> struct Program {
         return PO_OK;
     }
 
+    hipError_t launch(const Launch &l, int off, int n, const Shape &sh, const Roots<T> &rt, hipStream_t st) const {
+        return l.plane       ? launch_plane<T>(sh, rt, pd.p + off, n, st, plane_variant, split, hold)
+               : l.stage >= 0 ? launch_xetd<T>(sh, rt, ed.p + off, n, l.stage, st)
+                              : launch_xpass<T>(sh, rt, xd.p + off, n, st, x_variant, split);
+    }
+
+    // Two-lane programs with the lanes on two streams: every launch is split
+    // into its lane-0 and lane-1 halves (descriptors are stored lane by lane),
+    // so the two independent chains advance without waiting for each other
+    // and the GPU overlaps one chain's launches with the other's.  The caller
+    // orders s1 after s0's prior work and s0 after s1's last launch.  A fused
+    // adjoint program joins the streams once, before the second half of the
+    // contour, whose x-passes read the partner lane's stored spectra.
+    int run_dual(const Shape &sh, const Roots<T> &rt, hipStream_t s0, hipStream_t s1, hipEvent_t e0,
+                 hipEvent_t e1) const {
+        for (size_t i = 0; i < seq.size(); ++i) {
+            if ((int)i == mid_launch) {
+                PO_HIP(hipEventRecord(e0, s0));
+                PO_HIP(hipEventRecord(e1, s1));
+                PO_HIP(hipStreamWaitEvent(s0, e1, 0));
+                PO_HIP(hipStreamWaitEvent(s1, e0, 0));
+            }
+            const Launch &l = seq[i];
+            if (l.n % 2) return fail(PO_ERR_INVALID, "two-stream run of a launch with an odd descriptor count");
+            const int h = l.n / 2;
+            hipError_t e = launch(l, l.off, h, sh, rt, s0);
+            if (e == hipSuccess) e = launch(l, l.off + h, h, sh, rt, s1);
+            if (e != hipSuccess) return fail(PO_ERR_DEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
+        }
+        return PO_OK;
+    }
+
     int run(const Shape &sh, const Roots<T> &rt, hipStream_t st) const {
         for (const auto &l : seq) {
-            hipError_t e = l.plane       ? launch_plane<T>(sh, rt, pd.p + l.off, l.n, st, plane_variant, split, hold)
-                           : l.stage >= 0 ? launch_xetd<T>(sh, rt, ed.p + l.off, l.n, l.stage, st)
-                                          : launch_xpass<T>(sh, rt, xd.p + l.off, l.n, st, x_variant, split);
+            hipError_t e = launch(l, l.off, l.n, sh, rt, st);
             if (e != hipSuccess) return fail(PO_ERR_DEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
         }
         return PO_OK;
@@ -703,12 +738,16 @@ template <typename T> struct Program {
     // fixed, so one capture serves every later call.  Direct launches are the
     // default: replay measured no faster at 128^3 and direct launches are
     // what the profiler traces reliably.
-    int run_graph(const Shape &sh, const Roots<T> &rt, hipStream_t st) {
+    bool graph_on() {
         if (use_graph < 0) {
             const char *env = std::getenv("PO_GRAPH");
             use_graph = (env && env[0] == '1') ? 1 : 0;
         }
-        if (!use_graph) return run(sh, rt, st);
+        return use_graph == 1;
+    }
+
+    int run_graph(const Shape &sh, const Roots<T> &rt, hipStream_t st) {
+        if (!graph_on()) return run(sh, rt, st);
         if (!exec) {
             hipGraph_t g = nullptr;
             PO_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));

@@ -148,10 +148,14 @@ def test_cell_relaxation_reaches_brent_optimum(ctx64):
 
 @pytest.mark.parametrize("algo", ["RQM4", "OS2"])
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
-def test_fused_stress_matches_standalone(ctx64, ctx32, algo, precision):
+def test_fused_stress_matches_standalone(ctx64, ctx32, algo, precision, monkeypatch):
     """The stress fused into update() (spectra of the first half of the chains
     kept and dotted with the second half's) equals the standalone adjoint pass
-    at the same fields, and leaves the forward results bit-identical."""
+    at the same fields, and leaves the forward results bit-identical.  The
+    launch variants are pinned: each model tunes its own, and different
+    variants agree only to rounding."""
+    for k in ("PO_SPLIT", "PO_PLANE_VARIANT", "PO_XPASS_VARIANT", "PO_PLANE_HOLD"):
+        monkeypatch.setenv(k, "0")
     ctx = ctx64 if precision == "fp64" else ctx32
     shape, Ms = (32, 32, 32), (13, 19)
     lengths = (3.8, 4.0, 4.3)

@@ -188,6 +188,8 @@ def test_graph_replay_matches_direct_launches(ctx64, monkeypatch):
     """The propagator program replayed from a hipGraph (PO_GRAPH=1) gives the
     same iterations as direct launches, replay after replay, with host reads
     and synchronisations between the replays."""
+    for k in ("PO_SPLIT", "PO_PLANE_VARIANT", "PO_XPASS_VARIANT", "PO_PLANE_HOLD"):
+        monkeypatch.setenv(k, "0")  # same kernels in both models: variants agree only to rounding
     wl = workloads.cfg1(32)
     kw = wl.model_kwargs()
     direct = capi.ModelAB(ctx64, wl.wA, wl.wB, **kw)
@@ -239,3 +241,28 @@ def test_launch_variants_agree(ctx64, monkeypatch):
     H0, w0 = out[(0, 0, 0, 0)]
     for key, (H, w) in out.items():
         assert abs(H - H0) <= 1e-13 * abs(H0) and relerr(w, w0) <= 1e-13, key
+
+
+def test_two_stream_lanes_match_one_stream(ctx64, monkeypatch):
+    """The two chains on two streams (the default) give bit-identical
+    iterations and fused stress to both lanes in every launch on one stream
+    (PO_DUAL=0): the same kernels on the same descriptors, only scheduled
+    differently, with the fused adjoint's partner reads behind a join."""
+    for k in ("PO_SPLIT", "PO_PLANE_VARIANT", "PO_XPASS_VARIANT", "PO_PLANE_HOLD"):
+        monkeypatch.setenv(k, "0")
+    shape = (32, 32, 32)
+    wA = smooth_field(shape, 71) + 12.0
+    wB = smooth_field(shape, 72) + 6.0
+    kw = dict(cell=("ORTHORHOMBIC", 3.8, 4.0, 4.3), Ms=(13, 19), chiN=15.0, n_anderson=3)
+    two = capi.ModelAB(ctx64, wA, wB, **kw)
+    monkeypatch.setenv("PO_DUAL", "0")
+    one = capi.ModelAB(ctx64, wA, wB, **kw)
+    for m in (two, one):
+        m.set_adjoint(True)
+    for it in range(3):
+        two.update()
+        one.update()
+        a, b = two.observe(), one.observe()
+        assert a["H"] == b["H"] and a["residual"] == b["residual"], it
+        assert np.array_equal(two.last_stress(), one.last_stress()), it
+    assert np.array_equal(two.get("wA"), one.get("wA")) and np.array_equal(two.get("phiB"), one.get("phiB"))

@@ -338,11 +338,12 @@ __device__ __forceinline__ int plane_loc(int y, int c) {
 // k_xpass4, where both halves of a column meet.  Half planes take half the
 // LDS, so two workgroups share a CU and overlap their memory and LDS phases.
 // Half planes of up to 512 threads are meant to run two workgroups per CU:
-// their register budget is capped at that occupancy (except the FP64 hold
-// variant, which trades the second workgroup for the held slice copy).
+// their register budget is capped at that occupancy.  The FP64 hold variant
+// then spills a few registers (12 at 128 x 128), which costs less than the
+// second workgroup per CU would (134 vs 144 us per lockstep step at 128^3).
 template <typename T, int LYW, int LZ, int NTV, int SPLIT, int HOLD> struct PlaneOcc {
     static constexpr int NT = PlaneCfg<T, LYW, LZ, NTV>::NT;
-    static constexpr int value = (SPLIT && NT <= 512 && !(sizeof(T) == 8 && HOLD)) ? (2 * (NT / 64) + 3) / 4 : 1;
+    static constexpr int value = (SPLIT && NT <= 512) ? (2 * (NT / 64) + 3) / 4 : 1;
 };
 
 template <typename T, int LY, int LZ, int NTV, int SPLIT = 0, int HOLD = (sizeof(T) == 4)>

@@ -332,9 +332,7 @@ template <typename T> struct Model : po_model {
             PO_HIP(mark());
             Program<T> &pg = fused_on ? prog_adj : prog;
             int rc;
-            if (use_dual() && pg.graph_on()) {
-                rc = pg.run_dual_graph(sh, rt, st, st2, dev);
-            } else if (use_dual()) {
+            if (use_dual()) {
                 PO_HIP(hipEventRecord(dev[0], st));
                 PO_HIP(hipStreamWaitEvent(st2, dev[0], 0));
                 rc = pg.run_dual(sh, rt, st, st2, dev[1], dev[2]);
@@ -479,7 +477,8 @@ template <typename T> struct Model : po_model {
                 if (!ok) dual = 0;
             }
         }
-        return dual == 1 && lanes_.size() == 2;
+        Program<T> &pg = fused_on ? prog_adj : prog;
+        return dual == 1 && !pg.graph_on() && lanes_.size() == 2;
     }
     ~Model() override {
         if (st2) {

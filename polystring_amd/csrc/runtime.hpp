@@ -763,33 +763,8 @@ template <typename T> struct Program {
         return PO_OK;
     }
 
-    // run_dual captured once into a hipGraph (fork and join on s0 included)
-    // and replayed; ev: 4 events (fork, two mid-join, join).
-    int run_dual_graph(const Shape &sh, const Roots<T> &rt, hipStream_t s0, hipStream_t s1, hipEvent_t *ev) {
-        if (!exec2) {
-            hipGraph_t g = nullptr;
-            PO_HIP(hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal));
-            int rc = PO_OK;
-            hipError_t e = hipEventRecord(ev[0], s0);
-            if (e == hipSuccess) e = hipStreamWaitEvent(s1, ev[0], 0);
-            if (e == hipSuccess) rc = run_dual(sh, rt, s0, s1, ev[1], ev[2]);
-            if (e == hipSuccess && rc == PO_OK) e = hipEventRecord(ev[3], s1);
-            if (e == hipSuccess && rc == PO_OK) e = hipStreamWaitEvent(s0, ev[3], 0);
-            hipError_t ee = hipStreamEndCapture(s0, &g);
-            if (rc) return rc;
-            if (e != hipSuccess || ee != hipSuccess)
-                return fail(PO_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(e != hipSuccess ? e : ee));
-            e = hipGraphInstantiate(&exec2, g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            if (e != hipSuccess) return fail(PO_ERR_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(e));
-        }
-        PO_HIP(hipGraphLaunch(exec2, s0));
-        return PO_OK;
-    }
-
     ~Program() {
         if (exec) (void)hipGraphExecDestroy(exec);
-        if (exec2) (void)hipGraphExecDestroy(exec2);
     }
 
     long slice_elems = 0;  // M, set before build()
@@ -799,7 +774,7 @@ template <typename T> struct Program {
     bool has_tab = false;  // a full k table (general-cell OS2): no split-y kernel for it
     int hold = 0;          // FP64 half planes hold the second slice copy in registers
     int use_graph = -1;
-    hipGraphExec_t exec = nullptr, exec2 = nullptr;
+    hipGraphExec_t exec = nullptr;
 };
 
 }  // namespace po

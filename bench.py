@@ -253,7 +253,7 @@ def main():
                    "step": "one SCFT iteration (2*Ns contour steps)", "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": head["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": head["achieved_GBs"] / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "RQM4 lockstep contour step (k_plane3 + k_xpass3 launches advancing both chains"
+                     "kernel": "RQM4 lockstep contour step (k_plane3 + k_xpass3/k_xpass4 launches advancing both chains"
                                + (", with the fused stress adjoint)" if head is mixed else ")"),
                      "bytes_per_launch": head["bytes_per_launch"], "avg_launch_us": head["lockstep_step_us"]},
         "cpu_baseline": cpu,

@@ -299,6 +299,7 @@ template <typename T> struct Model : po_model {
         prog.slice_elems = M;
         lanes_ = lanes;
         if ((rc = prog.build(d.algo, lanes, st))) return rc;
+        if (use_dual()) return prog.tune(sh, rt, st, st2, dev);
         return prog.tune(sh, rt, st);
     }
 

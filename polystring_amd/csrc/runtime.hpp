@@ -655,7 +655,20 @@ template <typename T> struct Program {
     // and spectrum the program writes is rewritten by the next real run, so
     // tuning leaves no trace in the results.  All variants compute the same
     // arithmetic in the same order.
-    int tune(const Shape &sh, const Roots<T> &rt, hipStream_t st) {
+    // With st2 (and 4 events) the candidates are timed on the two-stream
+    // schedule the model will run (fork on st, lanes on st / st2, join).
+    int tune(const Shape &sh, const Roots<T> &rt, hipStream_t st, hipStream_t st2 = nullptr,
+             hipEvent_t *dev = nullptr) {
+        auto once = [&]() -> int {
+            if (!st2) return run(sh, rt, st);
+            PO_HIP(hipEventRecord(dev[0], st));
+            PO_HIP(hipStreamWaitEvent(st2, dev[0], 0));
+            int rc = run_dual(sh, rt, st, st2, dev[1], dev[2]);
+            if (rc) return rc;
+            PO_HIP(hipEventRecord(dev[3], st2));
+            PO_HIP(hipStreamWaitEvent(st, dev[3], 0));
+            return PO_OK;
+        };
         // PO_PLANE_VARIANT / PO_XPASS_VARIANT / PO_SPLIT / PO_PLANE_HOLD = 0|1
         // pin the choices (reproducible launch shapes) wherever the grid has
         // that kernel
@@ -713,10 +726,10 @@ template <typename T> struct Program {
             x_variant = c.x;
             split = c.s;
             hold = c.h;
-            rc = run(sh, rt, st);  // warm-up
+            rc = once();  // warm-up
             if (rc) return rc;
             PO_HIP(hipEventRecord(a, st));
-            for (int rep = 0; rep < 2 && rc == PO_OK; ++rep) rc = run(sh, rt, st);
+            for (int rep = 0; rep < 2 && rc == PO_OK; ++rep) rc = once();
             if (rc) return rc;
             PO_HIP(hipEventRecord(b, st));
             PO_HIP(hipEventSynchronize(b));

@@ -22,6 +22,7 @@ residual of that update, FP32 against FP64.
 Prints one JSON line.
 """
 import argparse
+import subprocess
 import json
 import os
 import sys
@@ -158,15 +159,44 @@ def cpu_baseline(seconds_target=15.0):
     ds = 1.0 / wl.Ns
     O.rqm4_solve(k2, ds, wl.wA, head, 2)  # plan + page-in
     # repeated 10-step solves (11 slices, 185 MB) until the sample reaches the target
-    nsteps, dt = 0, 0.0
-    t0 = time.perf_counter()
-    while dt < seconds_target:
-        O.rqm4_solve(k2, ds, wl.wA, head, 11)
-        nsteps += 10
-        dt = time.perf_counter() - t0
-    return dict(value=nsteps / dt, unit="contour-steps/s", cores=O.num_threads(), kind="port",
+    def sample(target):
+        nsteps, dt = 0, 0.0
+        t0 = time.perf_counter()
+        while dt < target:
+            O.rqm4_solve(k2, ds, wl.wA, head, 11)
+            nsteps += 10
+            dt = time.perf_counter() - t0
+        return nsteps, dt
+    nsteps, dt = sample(seconds_target)
+    cores = O.num_threads()
+    # the reference itself is single-threaded: the same sample on one thread
+    O.set_threads(1)
+    n1, dt1 = sample(seconds_target / 3)
+    O.set_threads(cores)
+    return dict(value=nsteps / dt, unit="contour-steps/s", cores=cores, kind="port",
                 sample=f"{nsteps} RQM4 contour steps of one propagator at 128^3 FP64 (oracle C restatement, "
-                       f"complex-to-complex FFTs as the reference), {dt:.1f} s on {O.num_threads()} threads")
+                       f"complex-to-complex FFTs as the reference), {dt:.1f} s on {cores} threads",
+                value_1core=n1 / dt1, sample_1core=f"{n1} contour steps, {dt1:.1f} s on 1 thread")
+
+
+def copy_bandwidth(nbytes=1 << 30, reps=5):
+    """Measured device copy rate (read + write bytes per second) of a plain
+    1 GiB device-to-device copy, timed with events: the practical HBM
+    ceiling on this box beside the 8 TB/s published peak."""
+    import torch
+    x = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    y.copy_(x)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        y.copy_(x)
+    b.record()
+    torch.cuda.synchronize()
+    gbs = 2.0 * nbytes * reps / (a.elapsed_time(b) * 1e-3) / 1e9
+    del x, y
+    torch.cuda.empty_cache()
+    return gbs
 
 
 def copy_probe(nbytes=1 << 30, reps=3):
@@ -186,6 +216,7 @@ def main():
 
     ap = argparse.ArgumentParser()
     ap.add_argument("--copy-probe", action="store_true", help="run only the traffic-calibration copy")
+    ap.add_argument("--copy-rate", action="store_true", help="print the measured device copy rate only")
     ap.add_argument("--form", choices=["both", "mixed", "fp64"], default="both",
                     help="time only one form (for profiling runs)")
     ap.add_argument("--gpus", type=int, default=1)
@@ -196,6 +227,9 @@ def main():
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
+        return
+    if args.copy_rate:
+        print(json.dumps({"copy_GBs": copy_bandwidth()}))
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (each GPU would run "
@@ -231,6 +265,10 @@ def main():
             "bytes_per_launch": etd["bytes_per_launch"], "H": etd["H"]}
         print(f"[bench] etdrk4 {json.dumps(extras['etdrk4_fp64'])}", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu_baseline else cpu_baseline()
+    # torch brings its own HIP runtime: the copy is timed in a child process
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--copy-rate"], capture_output=True, text=True,
+                       timeout=120)
+    copy_gbs = json.loads(r.stdout.strip().splitlines()[-1])["copy_GBs"] if r.returncode == 0 else None
 
     line = {
         "metric": "contour-steps/s at 128^3 Ns=200: mixed+adjoint vs FP64 plain, |dH/H|<=1e-6",
@@ -255,7 +293,9 @@ def main():
                      "frac": head["achieved_GBs"] / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "RQM4 lockstep contour step (k_plane3 + k_xpass3/k_xpass4 launches advancing both chains"
                                + (", with the fused stress adjoint)" if head is mixed else ")"),
-                     "bytes_per_launch": head["bytes_per_launch"], "avg_launch_us": head["lockstep_step_us"]},
+                     "bytes_per_launch": head["bytes_per_launch"], "avg_launch_us": head["lockstep_step_us"],
+                     "measured_copy_GBs": copy_gbs,
+                     "frac_of_measured_copy": head["achieved_GBs"] / copy_gbs if copy_gbs else None},
         "cpu_baseline": cpu,
         "iters_per_s": head["iters_per_s"],
     }

@@ -285,12 +285,11 @@ template <typename T> struct Model : po_model {
         lanes[1].Sf = S[2].p;
         lanes[1].Sh = S[3].p;
         if (d.algo == PO_ETDRK4) {
-            if ((rc = build_etd_tables<T>(etdA, sh, cell, dsA, st))) return rc;
-            if ((rc = build_etd_tables<T>(etdB, sh, cell, dsB, st))) return rc;
+            if ((rc = build_etd()) != PO_OK) return rc;
             const long ns = etd_slots<T>(sh);
             for (auto &b : etdState) PO_HIP(b.alloc(ns));
             for (int l = 0; l < 2; ++l) {
-                for (auto &sg : lanes[l].segs) sg.etd = sg.q == qA.p || sg.q == qAc.p ? &etdA : &etdB;
+                for (auto &sg : lanes[l].segs) sg.etd = (sg.q == qA.p || sg.q == qAc.p || dsA == dsB) ? &etdA : &etdB;
                 lanes[l].ev = etdState[3 * l].p;
                 lanes[l].eacc = etdState[3 * l + 1].p;
                 lanes[l].et = etdState[3 * l + 2].p;
@@ -318,6 +317,14 @@ template <typename T> struct Model : po_model {
         PO_HIP(launch_exp_factor<T>(wB.p, eB2.p, -0.5 * dsB, M, st));
         PO_HIP(launch_exp_factor<T>(wB.p, eB4.p, -0.25 * dsB, M, st));
         return PO_OK;
+    }
+
+    // ETDRK4 coefficient tables of both blocks; one set when dsA == dsB (the
+    // fork's ds = 1/Ns for both blocks, src/Model_AB.cc:100-111)
+    int build_etd() {
+        int rc = build_etd_tables<T>(etdA, sh, cell, dsA, ctx->stream);
+        if (rc == PO_OK && dsA != dsB) rc = build_etd_tables<T>(etdB, sh, cell, dsB, ctx->stream);
+        return rc;
     }
 
     int find_index(int j) const {  // src/FieldAX.cc:262-267
@@ -828,10 +835,7 @@ template <typename T> struct Model : po_model {
             if ((rc = build_ktable<T>(kAh, sh, cell, false, 0.5 * dsA, st))) return rc;
             if ((rc = build_ktable<T>(kBh, sh, cell, false, 0.5 * dsB, st))) return rc;
         }
-        if (desc.algo == PO_ETDRK4) {  // refilled in place: the program keeps its descriptors
-            if ((rc = build_etd_tables<T>(etdA, sh, cell, dsA, st))) return rc;
-            if ((rc = build_etd_tables<T>(etdB, sh, cell, dsB, st))) return rc;
-        }
+        if (desc.algo == PO_ETDRK4 && (rc = build_etd()) != PO_OK) return rc;  // refilled in place
         return restart_history();
     }
 

@@ -11,6 +11,7 @@
 #include <map>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/polystring_amd.h"
@@ -317,27 +318,36 @@ int build_etd_tables(EtdTables<T> &et, const Shape &sh, const Cell &cell, double
     std::vector<std::vector<T>> h(6, std::vector<T>((size_t)n));
     std::vector<std::complex<double>> r(npts);
     for (int j = 0; j < npts; ++j) r[j] = std::exp(std::complex<double>(0.0, pi * (j + 0.5) / npts));
-    for (int i = 0; i < sh.Lx; ++i)
-        for (int j = 0; j < sh.Ly; ++j)
-            for (int k = 0; k < sh.Hz; ++k) {
-                const double ri = rmin(i, sh.Lx), rj = rmin(j, sh.Ly), rk = rmin(k, sh.Lz);
-                const double L = cc[0] * ri * ri + cc[1] * rj * rj + cc[2] * rk * rk;
-                double sq = 0, s1 = 0, s2 = 0, s3 = 0;
-                for (int m = 0; m < npts; ++m) {
-                    const std::complex<double> z = ds * L + r[m], ez = std::exp(z), z2 = z * z, z3 = z2 * z;
-                    sq += std::real((std::exp(z / 2.0) - 1.0) / z);
-                    s1 += std::real((ez * (z2 - 3.0 * z + 4.0) - z - 4.0) / z3);
-                    s2 += std::real((ez * (z - 2.0) + z + 2.0) / z3);
-                    s3 += std::real((ez * (4.0 - z) - z2 - 3.0 * z - 4.0) / z3);
+    // x planes split over host threads (disjoint outputs)
+    auto planes = [&](int i0, int i1) {
+        for (int i = i0; i < i1; ++i)
+            for (int j = 0; j < sh.Ly; ++j)
+                for (int k = 0; k < sh.Hz; ++k) {
+                    const double ri = rmin(i, sh.Lx), rj = rmin(j, sh.Ly), rk = rmin(k, sh.Lz);
+                    const double L = cc[0] * ri * ri + cc[1] * rj * rj + cc[2] * rk * rk;
+                    double sq = 0, s1 = 0, s2 = 0, s3 = 0;
+                    for (int m = 0; m < npts; ++m) {
+                        const std::complex<double> z = ds * L + r[m], ez = std::exp(z), z2 = z * z, z3 = z2 * z;
+                        sq += std::real((std::exp(z / 2.0) - 1.0) / z);
+                        s1 += std::real((ez * (z2 - 3.0 * z + 4.0) - z - 4.0) / z3);
+                        s2 += std::real((ez * (z - 2.0) + z + 2.0) / z3);
+                        s3 += std::real((ez * (4.0 - z) - z2 - 3.0 * z - 4.0) / z3);
+                    }
+                    const size_t o = ((size_t)i * sh.Ly + j) * sh.Hz + k;
+                    h[0][o] = (T)std::exp(ds * L);
+                    h[1][o] = (T)std::exp(ds * L / 2.0);
+                    h[2][o] = (T)(ds * sq / npts);
+                    h[3][o] = (T)(ds * s1 / npts);
+                    h[4][o] = (T)(ds * s2 / npts);
+                    h[5][o] = (T)(ds * s3 / npts);
                 }
-                const size_t o = ((size_t)i * sh.Ly + j) * sh.Hz + k;
-                h[0][o] = (T)std::exp(ds * L);
-                h[1][o] = (T)std::exp(ds * L / 2.0);
-                h[2][o] = (T)(ds * sq / npts);
-                h[3][o] = (T)(ds * s1 / npts);
-                h[4][o] = (T)(ds * s2 / npts);
-                h[5][o] = (T)(ds * s3 / npts);
-            }
+    };
+    {
+        const int nth = std::max(1, std::min({sh.Lx, (int)std::thread::hardware_concurrency(), 16}));
+        std::vector<std::thread> pool;
+        for (int t = 0; t < nth; ++t) pool.emplace_back(planes, sh.Lx * t / nth, sh.Lx * (t + 1) / nth);
+        for (auto &th : pool) th.join();
+    }
     const long slots = etd_slots<T>(sh);
     if (slots < n) return fail(PO_ERR_INVALID, "grid has no ETDRK4 x-pencil layout");
     DBuf<T> tmp;

@@ -132,17 +132,21 @@ template <typename T> hipError_t launch_mean_partials(const T *x, long M, double
 
 // out[v] = (sum_b partials[b * nval + v]) / div, summed in block order; with
 // div = M this is blitz::mean (src/Grid.cc:329-338).
+// One wave per value: lane l sums blocks l, l + 64, ... in order, then a
+// fixed shuffle tree -- the same order on every call (deterministic).
 __global__ void k_finish_partials(const double *__restrict__ partials, int nblk, int nval, double div, double *out) {
-    for (int v = threadIdx.x; v < nval; v += blockDim.x) {
+    const int lane = threadIdx.x & 63;
+    for (int v = threadIdx.x >> 6; v < nval; v += blockDim.x >> 6) {
         double s = 0.0;
-        for (int b = 0; b < nblk; ++b) s += partials[(long)b * nval + v];
-        out[v] = s / div;
+        for (int b = lane; b < nblk; b += 64) s += partials[(long)b * nval + v];
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+        if (lane == 0) out[v] = s / div;
     }
 }
 
 hipError_t launch_finish_partials(const double *partials, int nblk, int nval, double div, double *out,
                                   hipStream_t st) {
-    hipLaunchKernelGGL(k_finish_partials, dim3(1), dim3(64), 0, st, partials, nblk, nval, div, out);
+    hipLaunchKernelGGL(k_finish_partials, dim3(1), dim3(256), 0, st, partials, nblk, nval, div, out);
     return hipGetLastError();
 }
 
@@ -397,9 +401,12 @@ k_anderson_mix(AndersonArgs a, const double *__restrict__ C, double *__restrict_
 
 // H = Hw + Hs with Hs = -log Q (src/Model_AB.cc:294-300)
 __global__ void k_observables(const double *__restrict__ partials, int nblk, long M, double *obs) {
+    // one wave: lane l sums blocks l, l + 64, ... in order, then a fixed tree
+    const int lane = threadIdx.x & 63;
+    double s = 0.0;
+    for (int b = lane; b < nblk; b += 64) s += partials[b];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
     if (threadIdx.x == 0) {
-        double s = 0.0;
-        for (int b = 0; b < nblk; ++b) s += partials[b];
         const double hw = s / (double)M;
         const double hs = -log(obs[OBS_Q]);
         obs[OBS_HW] = hw;

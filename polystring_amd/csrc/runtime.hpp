@@ -385,6 +385,7 @@ template <typename T> struct Program {
         bool plane;
         int off, n;
         int stage = -1;  // >= 0: ETDRK4 x-pencil stage (descriptors in ed)
+        int adj = 1;     // x-pencil launch: some descriptor stores or dots adjoint spectra
     };
     DBuf<PlaneDesc<T>> pd;
     DBuf<XDesc<T>> xd;
@@ -423,7 +424,12 @@ template <typename T> struct Program {
             while (s >= lanes[lane].segs[si].len - 1) { s -= lanes[lane].segs[si].len - 1; ++si; }
         };
         auto push_p = [&](std::vector<PlaneDesc<T>> &v) { seq.push_back({true, (int)P.size(), (int)v.size()}); P.insert(P.end(), v.begin(), v.end()); };
-        auto push_x = [&](std::vector<XDesc<T>> &v) { seq.push_back({false, (int)X.size(), (int)v.size()}); X.insert(X.end(), v.begin(), v.end()); };
+        auto push_x = [&](std::vector<XDesc<T>> &v) {
+            int a = 0;
+            for (const auto &x : v) a |= (x.Sstore || x.Sdot) ? 1 : 0;
+            seq.push_back({false, (int)X.size(), (int)v.size(), -1, a});
+            X.insert(X.end(), v.begin(), v.end());
+        };
         const bool rqm4 = algo == PO_RQM4;
         if (adj && (L != 2 || nsteps % 2 != 0))
             return fail(PO_ERR_INVALID, "the fused adjoint needs two lanes with an even step count");
@@ -620,7 +626,7 @@ template <typename T> struct Program {
     hipError_t launch(const Launch &l, int off, int n, const Shape &sh, const Roots<T> &rt, hipStream_t st) const {
         return l.plane       ? launch_plane<T>(sh, rt, pd.p + off, n, st, plane_variant, split, hold)
                : l.stage >= 0 ? launch_xetd<T>(sh, rt, ed.p + off, n, l.stage, st)
-                              : launch_xpass<T>(sh, rt, xd.p + off, n, st, x_variant, split);
+                              : launch_xpass<T>(sh, rt, xd.p + off, n, st, x_variant, split, l.adj);
     }
 
     // Two-lane programs with the lanes on two streams: every launch is split

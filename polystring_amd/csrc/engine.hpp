@@ -136,11 +136,11 @@ bool xpass_supported(int Lx);
 template <typename T>
 hipError_t launch_plane(const Shape &sh, const Roots<T> &rt, const PlaneDesc<T> *d_desc, int nlanes,
                         hipStream_t st, int variant = 0, int split = 0, int hold = 0);
-// adj = 0: no descriptor of the launch carries a fused-adjoint store or dot
-// (the kernel is compiled without them)
+// adj: what the launch's descriptors carry of the fused adjoint -- 0 nothing,
+// 1 spectrum stores only, 2 dots (and stores); the kernel is compiled for it
 template <typename T>
 hipError_t launch_xpass(const Shape &sh, const Roots<T> &rt, const XDesc<T> *d_desc, int nlanes,
-                        hipStream_t st, int xvariant = 0, int split = 0, int adj = 1);
+                        hipStream_t st, int xvariant = 0, int split = 0, int adj = 2);
 bool split_supported(const Shape &sh);
 
 // ETDRK4 x-pencil stage (see EtdDesc); stage in [0, 4).

@@ -725,9 +725,10 @@ template <typename T, int LX, int W = 0> struct X3Cfg {
     static __device__ __forceinline__ int loc(int x, int c) { return x * RS + (SH ? (x >> SH) * PADX : 0) + c; }
 };
 
-// ADJ = 0 compiles the fused adjoint store / dot out (launches of programs
-// without the adjoint: fewer registers, higher occupancy).
-template <typename T, int LX, int W, int ADJ = 1>
+// ADJ: 0 compiles the fused adjoint out (programs without it), 1 keeps only
+// the spectrum store (first half of the contour), 2 keeps store and dot:
+// fewer registers and higher occupancy where the launch needs less.
+template <typename T, int LX, int W, int ADJ = 2>
 __global__ void __launch_bounds__((X3Cfg<T, LX, W>::NT))
 k_xpass3(const XDesc<T> *__restrict__ descs, Roots<T> rt, int ncol, int hz) {
     using X = X3Cfg<T, LX, W>;
@@ -751,7 +752,7 @@ k_xpass3(const XDesc<T> *__restrict__ descs, Roots<T> rt, int ncol, int hz) {
     // full 3-D spectrum of this wave's columns, in register order (stored
     // for, or read from, the partner chain of the adjoint stress)
     const unsigned sbase = (blockIdx.x * X::NW + w) * (unsigned)(ITX * RX * 64);
-    if (ADJ && d.Sstore) {
+    if (ADJ >= 1 && d.Sstore) {
 #pragma unroll
         for (int it = 0; it < ITX; ++it)
             if ((TOTX % 64 == 0) || lane + it * 64 < TOTX)
@@ -769,7 +770,7 @@ k_xpass3(const XDesc<T> *__restrict__ descs, Roots<T> rt, int ncol, int hz) {
                 const int x = j + r * NBX;
                 const T f = d.tab ? gload(d.tab + (unsigned)(x * ncol + c))
                                   : gload(d.fx + x) * gload(d.fy + c / hz) * gload(d.fz + c % hz);
-                if (ADJ && d.Sdot) {
+                if (ADJ == 2 && d.Sdot) {
                     // sum_k w(kz) Re(conj(Y) X) K dK/dl_d / K, X = this chain, Y = partner
                     const cx<T> y = gload(d.Sdot + sbase + (unsigned)((it * RX + r) * 64 + lane));
                     const int ky = c / hz, kz = c % hz;
@@ -788,7 +789,7 @@ k_xpass3(const XDesc<T> *__restrict__ descs, Roots<T> rt, int ncol, int hz) {
         WaveFFT<T, LX, true, CPW, true, IO_REG, IO_GLB>::run(lds, 1, RS, rt.x, lane, nullptr, 1, ncol,
                                                              (d.Sout ? d.Sout : d.S) + cw, v, b, xmap);
     }
-    if (ADJ && d.Sdot) {
+    if (ADJ == 2 && d.Sdot) {
         // workgroup sum of the dot, through the (now free) transform buffer,
         // so the kernel needs no LDS beyond it
         double *red = reinterpret_cast<double *>(buf);
@@ -815,7 +816,7 @@ k_xpass3(const XDesc<T> *__restrict__ descs, Roots<T> rt, int ncol, int hz) {
 // with t = W_Ly^k' (ky = k' and k' + Ly/2), K and the fused adjoint on the
 // full spectrum, the inverse stage E' = Y_lo + Y_hi, O' = (Y_lo - Y_hi) t*,
 // and inverse x transforms.  Needs Ly Hz / 2 to be a multiple of C.
-template <typename T, int LX, int W, int ADJ = 1>
+template <typename T, int LX, int W, int ADJ = 2>
 __global__ void __launch_bounds__((X3Cfg<T, LX, W>::NT))
 k_xpass4(const XDesc<T> *__restrict__ descs, Roots<T> rt, int ncol, int hz, int lyh) {
     using X = X3Cfg<T, LX, W>;
@@ -860,13 +861,13 @@ k_xpass4(const XDesc<T> *__restrict__ descs, Roots<T> rt, int ncol, int hz, int 
                 const cx<T> to = cmul<false, T>(vo[it][r], t);
                 cx<T> lo = add<T>(ve[it][r], to), hi = sub<T>(ve[it][r], to);
                 const unsigned so = sbase + (unsigned)((it * RX + r) * 64 + lane);
-                if (ADJ && d.Sstore) {
+                if (ADJ >= 1 && d.Sstore) {
                     gstore(d.Sstore + so, lo);
                     gstore(d.Sstore + so + ITX * RX * 64, hi);
                 }
                 const T fx = gload(d.fx + x);
                 const T f0 = fx * fy0 * fz, f1 = fx * fy1 * fz;
-                if (ADJ && d.Sdot) {
+                if (ADJ == 2 && d.Sdot) {
                     const cx<T> y0 = gload(d.Sdot + so), y1 = gload(d.Sdot + so + ITX * RX * 64);
                     const double p0 = wz * (double)f0 * ((double)y0.x * (double)lo.x + (double)y0.y * (double)lo.y);
                     const double p1 = wz * (double)f1 * ((double)y1.x * (double)hi.x + (double)y1.y * (double)hi.y);
@@ -892,7 +893,7 @@ k_xpass4(const XDesc<T> *__restrict__ descs, Roots<T> rt, int ncol, int hz, int 
         WaveFFT<T, LX, true, CPW, true, IO_REG, IO_GLB>::run(lds, 1, RS, rt.x, lane, nullptr, 1, ncol,
                                                              (d.Sout ? d.Sout : d.S) + half + cw, vo, bb, xmap);
     }
-    if (ADJ && d.Sdot) {
+    if (ADJ == 2 && d.Sdot) {
         double *red = reinterpret_cast<double *>(buf);
 #pragma unroll
         for (int k = 0; k < 3; ++k)
@@ -1223,6 +1224,20 @@ hipError_t launch_x3(const Shape &sh, const Roots<T> &rt, const XDesc<T> *d_desc
     return hipGetLastError();
 }
 
+template <typename T, int a, int W, bool SPL>
+hipError_t launch_xa(const Shape &sh, const Roots<T> &rt, const XDesc<T> *d_desc, int nlanes, hipStream_t st,
+                     int adj) {
+    if constexpr (SPL) {
+        if (adj == 0) return launch_x4<T, a, W, 0>(sh, rt, d_desc, nlanes, st);
+        if (adj == 1) return launch_x4<T, a, W, 1>(sh, rt, d_desc, nlanes, st);
+        return launch_x4<T, a, W, 2>(sh, rt, d_desc, nlanes, st);
+    } else {
+        if (adj == 0) return launch_x3<T, a, W, 0>(sh, rt, d_desc, nlanes, st);
+        if (adj == 1) return launch_x3<T, a, W, 1>(sh, rt, d_desc, nlanes, st);
+        return launch_x3<T, a, W, 2>(sh, rt, d_desc, nlanes, st);
+    }
+}
+
 template <typename T>
 hipError_t launch_xpass(const Shape &sh, const Roots<T> &rt, const XDesc<T> *d_desc, int nlanes,
                         hipStream_t st, int xvariant, int split, int adj) {
@@ -1232,17 +1247,12 @@ hipError_t launch_xpass(const Shape &sh, const Roots<T> &rt, const XDesc<T> *d_d
     if (sh.Lx == a) {                                                                           \
         if (split) {                                                                            \
             if (xvariant == 1 && half % X3Cfg<T, a, 1>::C == 0)                                 \
-                return adj ? launch_x4<T, a, 1, 1>(sh, rt, d_desc, nlanes, st)                  \
-                           : launch_x4<T, a, 1, 0>(sh, rt, d_desc, nlanes, st);                 \
-            return adj ? launch_x4<T, a, 0, 1>(sh, rt, d_desc, nlanes, st)                      \
-                       : launch_x4<T, a, 0, 0>(sh, rt, d_desc, nlanes, st);                     \
+                return launch_xa<T, a, 1, true>(sh, rt, d_desc, nlanes, st, adj);              \
+            return launch_xa<T, a, 0, true>(sh, rt, d_desc, nlanes, st, adj);                  \
         }                                                                                       \
         if (xvariant == 1 && ncol % X3Cfg<T, a, 1>::C == 0)                                     \
-            return adj ? launch_x3<T, a, 1, 1>(sh, rt, d_desc, nlanes, st)                      \
-                       : launch_x3<T, a, 1, 0>(sh, rt, d_desc, nlanes, st);                     \
-        if (ncol % X3Cfg<T, a>::C == 0)                                                         \
-            return adj ? launch_x3<T, a, 0, 1>(sh, rt, d_desc, nlanes, st)                      \
-                       : launch_x3<T, a, 0, 0>(sh, rt, d_desc, nlanes, st);                     \
+            return launch_xa<T, a, 1, false>(sh, rt, d_desc, nlanes, st, adj);                 \
+        if (ncol % X3Cfg<T, a>::C == 0) return launch_xa<T, a, 0, false>(sh, rt, d_desc, nlanes, st, adj); \
         const int nt = (ncol + XCfg<T, a>::C - 1) / XCfg<T, a>::C;                              \
         hipLaunchKernelGGL((k_xpass<T, a>), dim3(nt, nlanes), dim3(XCfg<T, a>::NT), 0, st, d_desc, rt, ncol, \
                            sh.Hz);                                                              \

@@ -385,7 +385,7 @@ template <typename T> struct Program {
         bool plane;
         int off, n;
         int stage = -1;  // >= 0: ETDRK4 x-pencil stage (descriptors in ed)
-        int adj = 1;     // x-pencil launch: some descriptor stores or dots adjoint spectra
+        int adj = 2;     // x-pencil launch: 0 no adjoint, 1 spectrum stores only, 2 dots
     };
     DBuf<PlaneDesc<T>> pd;
     DBuf<XDesc<T>> xd;
@@ -426,7 +426,7 @@ template <typename T> struct Program {
         auto push_p = [&](std::vector<PlaneDesc<T>> &v) { seq.push_back({true, (int)P.size(), (int)v.size()}); P.insert(P.end(), v.begin(), v.end()); };
         auto push_x = [&](std::vector<XDesc<T>> &v) {
             int a = 0;
-            for (const auto &x : v) a |= (x.Sstore || x.Sdot) ? 1 : 0;
+            for (const auto &x : v) a = std::max(a, x.Sdot ? 2 : x.Sstore ? 1 : 0);
             seq.push_back({false, (int)X.size(), (int)v.size(), -1, a});
             X.insert(X.end(), v.begin(), v.end());
         };

@@ -44,6 +44,9 @@ def run_form(precision, steps, warmup, adjoint, L=128, Ns=200, algo="RQM4"):
     m = capi.ModelAB(ctx, wl.wA, wl.wB, algo=algo, **wl.model_kwargs())
     if adjoint:
         m.set_adjoint(True)
+    shape = m.launch_shape()
+    print(f"[bench] launch shape {precision}: " + " ".join(f"{k}={v}" for k, v in shape.items()), file=sys.stderr,
+          flush=True)
     # first update from the initial fields: the same-input precision probe.
     # H_fixed is H at the fields the propagators solved with,
     # mean(chiN phiA phiB - wA phiA - wB phiB) - ln Q (src/Model_AB.cc:263-315),
@@ -87,7 +90,7 @@ def run_form(precision, steps, warmup, adjoint, L=128, Ns=200, algo="RQM4"):
         contour_steps_per_s=cps * steps / wall, iters_per_s=steps / wall,
         propagator_ms=prof["prop_ms"], density_anderson_ms=prof["rest_ms"],
         lockstep_step_us=step_ms * 1e3, achieved_GBs=achieved, bytes_per_launch=bytes_launch,
-        H=obs["H"], residual=obs["residual"], Q=obs["Q"], stress=stress,
+        H=obs["H"], residual=obs["residual"], Q=obs["Q"], stress=stress, launch_shape=shape,
     )
     del m
     ctx.close()
@@ -288,7 +291,8 @@ def main():
                                ("FP32 propagators with FP64 Q/H/Anderson and the fused cell-stress adjoint"
                                 if head is mixed else "FP64 plain"),
                    "grid": [128, 128, 128], "Ns": 200, "Ms": [71, 131],
-                   "step": "one SCFT iteration (2*Ns contour steps)", "parallelism": "single GPU"},
+                   "step": "one SCFT iteration (2*Ns contour steps)", "parallelism": "single GPU",
+                   "launch_shape": head["launch_shape"]},
         "roofline": {"bound": "hbm", "achieved": head["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": head["achieved_GBs"] / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "RQM4 lockstep contour step (k_plane3 + k_xpass3/k_xpass4 launches advancing both chains"

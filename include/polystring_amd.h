@@ -243,6 +243,12 @@ long po_string_contour_steps(const po_string *s);
  * qBc|qAc by one contour step).  Switching profiling on resets the totals. */
 int po_model_set_profiling(po_model *m, int on);
 int po_model_profile(po_model *m, double *out4);
+/* The launch shape the autotuner picked for the model's propagator program
+ * (no counterpart in the reference): out5 = {split-y layout, plane-kernel
+ * variant, x-pencil variant, FP64 held slice copy, two-stream schedule}, the
+ * values the PO_SPLIT / PO_PLANE_VARIANT / PO_XPASS_VARIANT / PO_PLANE_HOLD /
+ * PO_DUAL variables pin. */
+int po_model_launch_shape(po_model *m, int *out5);
 
 #ifdef __cplusplus
 }

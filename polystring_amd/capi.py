@@ -45,7 +45,7 @@ EXPORTS = [
     "po_model_relax_cell", "po_model_set_adjoint", "po_model_last_stress", "po_model_grad_w",
     "po_model_set_fields", "po_string_create", "po_string_destroy", "po_string_iterate", "po_string_get",
     "po_string_contour_steps", "po_model_set_graph", "po_models_relax",
-    "po_model_set_profiling", "po_model_profile",
+    "po_model_set_profiling", "po_model_profile", "po_model_launch_shape",
 ]
 
 
@@ -113,6 +113,7 @@ def load():
         "po_model_set_graph": ([_vp, i], i),
         "po_models_relax": ([ctypes.POINTER(_vp), i, i, i, i, d, ctypes.POINTER(i), _dp, _dp], i),
         "po_model_set_profiling": ([_vp, i], i), "po_model_profile": ([_vp, _dp], i),
+        "po_model_launch_shape": ([_vp, ctypes.POINTER(ctypes.c_int)], i),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -318,6 +319,12 @@ class ModelAB:
 
     def set_profiling(self, on=True):
         _check(load().po_model_set_profiling(self.h, 1 if on else 0))
+
+    def launch_shape(self):
+        """The autotuned launch shape (and the PO_* variables that pin it)."""
+        o = (ctypes.c_int * 5)()
+        _check(load().po_model_launch_shape(self.h, o))
+        return dict(PO_SPLIT=o[0], PO_PLANE_VARIANT=o[1], PO_XPASS_VARIANT=o[2], PO_PLANE_HOLD=o[3], PO_DUAL=o[4])
 
     def profile(self):
         o = np.zeros(4)

@@ -175,6 +175,7 @@ struct po_model {
     virtual int set_fields(const double *wA, const double *wB) = 0;
     virtual int last_stress(double *dHdl) = 0;
     virtual int set_cell(const double *lengths_angles) = 0;
+    virtual void launch_shape(int *out5) = 0;
 };
 
 namespace po {
@@ -466,6 +467,14 @@ template <typename T> struct Model : po_model {
     void set_graph(bool on) override {
         prog.use_graph = on ? 1 : 0;
         prog_adj.use_graph = on ? 1 : 0;
+    }
+
+    void launch_shape(int *o) override {
+        o[0] = prog.split;
+        o[1] = prog.plane_variant;
+        o[2] = prog.x_variant;
+        o[3] = prog.hold;
+        o[4] = use_dual() ? 1 : 0;
     }
 
     // The two chains on two streams (Program::run_dual), the default for
@@ -1240,6 +1249,12 @@ int po_model_set_profiling(po_model *m, int on) {
     m->profiling = on != 0;
     m->ev_used = 0;
     m->prof_iters = 0;
+    return PO_OK;
+}
+
+int po_model_launch_shape(po_model *m, int *out5) {
+    if (!m || !out5) return fail(PO_ERR_INVALID, "null argument");
+    m->launch_shape(out5);
     return PO_OK;
 }
 

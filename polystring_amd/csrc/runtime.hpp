@@ -377,10 +377,10 @@ template <typename T> struct AdjointArena {
 };
 
 template <typename T> struct Program {
-    // FP64 takes exp(-w ds/2) as the square of exp(-w ds/4) (two fewer
-    // slice reads per step, a rounding-level change); FP32 keeps both tables,
-    // whose independent roundings hold the mixed-precision error of H lower.
-    static constexpr int kE2Sq = sizeof(T) == 8 ? 1 : 0;
+    // exp(-w ds/2) is taken as the square of exp(-w ds/4): two fewer slice
+    // reads per step, a rounding-level change (FP32 at 128^3: |dH/H| at fixed
+    // fields 1e-7 against FP64, from 2e-7 with both tables).
+    static constexpr int kE2Sq = 1;
     struct Launch {
         bool plane;
         int off, n;

@@ -435,7 +435,13 @@ template <typename T> struct Model : po_model {
             return PO_OK;
         }
         const bool rqm4 = desc.algo == PO_RQM4;
-        const long elems = xstore_elems<T>(sh, prog.x_variant, prog.split);
+        // The fused program keeps the plain program's tuned plane shape but
+        // uses the 16-column x tiles unless PO_XPASS_VARIANT pins the tile:
+        // with the adjoint dot compiled in, the 32-column kernel needs far more
+        // registers (FP32 234 vs 150 VGPRs), which the plain program's
+        // timing does not see.
+        const int xv = std::getenv("PO_XPASS_VARIANT") ? prog.x_variant : 0;
+        const long elems = xstore_elems<T>(sh, xv, prog.split);
         const int N = (sA - 1) + (sB - 1);
         if (elems < 0 || N % 2 != 0 || (!rqm4 && !cell.ortho))
             return fail(PO_ERR_INVALID, "the fused adjoint needs Ly*(Lz/2+1) divisible by 16, an even number of "
@@ -444,7 +450,7 @@ template <typename T> struct Model : po_model {
         const int nk = rqm4 ? 3 : 1;
         AdjointArena<T> a;
         a.elems = elems;
-        a.nblk = xpass_blocks(sh, prog.x_variant, prog.split);
+        a.nblk = xpass_blocks(sh, xv, prog.split);
         PO_HIP(fusedSpec.alloc((size_t)(N / 2) * 2 * nk * elems));
         PO_HIP(fusedPart.alloc((size_t)(N / 2) * 2 * nk * a.nblk * 3));
         PO_HIP(fusedSum.alloc(3));
@@ -457,7 +463,7 @@ template <typename T> struct Model : po_model {
             return rc;
         }
         prog_adj.plane_variant = prog.plane_variant;
-        prog_adj.x_variant = prog.x_variant;
+        prog_adj.x_variant = xv;
         prog_adj.split = prog.split;
         prog_adj.hold = prog.hold;
         fused_on = true;
@@ -470,10 +476,11 @@ template <typename T> struct Model : po_model {
     }
 
     void launch_shape(int *o) override {
-        o[0] = prog.split;
-        o[1] = prog.plane_variant;
-        o[2] = prog.x_variant;
-        o[3] = prog.hold;
+        const Program<T> &pg = fused_on ? prog_adj : prog;
+        o[0] = pg.split;
+        o[1] = pg.plane_variant;
+        o[2] = pg.x_variant;
+        o[3] = pg.hold;
         o[4] = use_dual() ? 1 : 0;
     }
 

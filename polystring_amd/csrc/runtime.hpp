@@ -744,13 +744,17 @@ template <typename T> struct Program {
             hold = c.h;
             rc = once();  // warm-up
             if (rc) return rc;
-            PO_HIP(hipEventRecord(a, st));
-            for (int rep = 0; rep < 2 && rc == PO_OK; ++rep) rc = once();
-            if (rc) return rc;
-            PO_HIP(hipEventRecord(b, st));
-            PO_HIP(hipEventSynchronize(b));
-            float ms = 0.f;
-            PO_HIP(hipEventElapsedTime(&ms, a, b));
+            // fastest of three single runs: robust to a disturbed run
+            float ms = 1e30f;
+            for (int rep = 0; rep < 3; ++rep) {
+                PO_HIP(hipEventRecord(a, st));
+                if ((rc = once())) return rc;
+                PO_HIP(hipEventRecord(b, st));
+                PO_HIP(hipEventSynchronize(b));
+                float t = 0.f;
+                PO_HIP(hipEventElapsedTime(&t, a, b));
+                ms = std::min(ms, t);
+            }
             if (ms < best) { best = ms; pick = c; }
         }
         (void)hipEventDestroy(a);

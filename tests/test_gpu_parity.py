@@ -266,3 +266,21 @@ def test_two_stream_lanes_match_one_stream(ctx64, monkeypatch):
         assert a["H"] == b["H"] and a["residual"] == b["residual"], it
         assert np.array_equal(two.last_stress(), one.last_stress()), it
     assert np.array_equal(two.get("wA"), one.get("wA")) and np.array_equal(two.get("phiB"), one.get("phiB"))
+
+
+def test_launch_shape_reports_pins(ctx64, monkeypatch):
+    """po_model_launch_shape returns the tuned shape, and pinned variables are
+    what it reports (where the grid has the kernels)."""
+    wl = workloads.cfg1(32)
+    kw = wl.model_kwargs()
+    m = capi.ModelAB(ctx64, wl.wA, wl.wB, **kw)
+    shape = m.launch_shape()
+    assert set(shape) == {"PO_SPLIT", "PO_PLANE_VARIANT", "PO_XPASS_VARIANT", "PO_PLANE_HOLD", "PO_DUAL"}
+    for k, v in shape.items():
+        monkeypatch.setenv(k, str(v))
+    again = capi.ModelAB(ctx64, wl.wA, wl.wB, **kw)
+    assert again.launch_shape() == shape
+    for a, b in ((m, again),):
+        a.update(2)
+        b.update(2)
+        assert a.observe()["H"] == b.observe()["H"]

@@ -52,6 +52,23 @@ __device__ __forceinline__ void gstore(float2 *p, float2 v) {
     q[0] = v.x;
     q[1] = v.y;
 }
+// Streaming (nontemporal) forms for data no launch re-reads soon -- stored
+// contour slices and the fused adjoint's spectra -- so they do not evict
+// the spectra and factor tables the next launch reads from the cache.
+template <typename V> __device__ __forceinline__ void gstore_nt(V *p, V v) {
+    using E = decltype(v.x);
+    PO_GLOBAL E *q = (PO_GLOBAL E *)p;
+    __builtin_nontemporal_store(v.x, q);
+    __builtin_nontemporal_store(v.y, q + 1);
+}
+template <typename V> __device__ __forceinline__ V gload_nt(const V *p) {
+    using E = decltype(p->x);
+    const PO_GLOBAL E *q = (const PO_GLOBAL E *)p;
+    V v;
+    v.x = __builtin_nontemporal_load(q);
+    v.y = __builtin_nontemporal_load(q + 1);
+    return v;
+}
 template <typename T> __device__ __forceinline__ cx<T> add(cx<T> a, cx<T> b) { return mk<T>(a.x + b.x, a.y + b.y); }
 template <typename T> __device__ __forceinline__ cx<T> sub(cx<T> a, cx<T> b) { return mk<T>(a.x - b.x, a.y - b.y); }
 template <typename T> __device__ __forceinline__ cx<T> scl(cx<T> a, T s) { return mk<T>(a.x * s, a.y * s); }

@@ -506,8 +506,8 @@ k_plane3(const PlaneDesc<T> *__restrict__ descs, Roots<T> rt, T invM) {
                         if constexpr (kHold)
                             if (held1) qh[it][r] = mk<T>(f.x * q[it][r].x, f.y * q[it][r].y);
                         if (fuse0) {
-                            gstore(const_cast<cx<T> *>(pair(d.qout, it, r)), q[it][r]);
-                            if (d.qout2) gstore(const_cast<cx<T> *>(pair(d.qout2, it, r)), q[it][r]);
+                            gstore_nt(const_cast<cx<T> *>(pair(d.qout, it, r)), q[it][r]);
+                            if (d.qout2) gstore_nt(const_cast<cx<T> *>(pair(d.qout2, it, r)), q[it][r]);
                             const cx<T> f2 = sq(f);
                             q[it][r] = mk<T>(f2.x * q[it][r].x, f2.y * q[it][r].y);
                         }
@@ -529,8 +529,8 @@ k_plane3(const PlaneDesc<T> *__restrict__ descs, Roots<T> rt, T invM) {
             if (valid(it)) {
 #pragma unroll
                 for (int r = 0; r < RZ; ++r) {
-                    gstore(const_cast<cx<T> *>(pair(d.qout, it, r)), q[it][r]);
-                    if (d.qout2) gstore(const_cast<cx<T> *>(pair(d.qout2, it, r)), q[it][r]);
+                    gstore_nt(const_cast<cx<T> *>(pair(d.qout, it, r)), q[it][r]);
+                    if (d.qout2) gstore_nt(const_cast<cx<T> *>(pair(d.qout2, it, r)), q[it][r]);
                 }
             }
     }
@@ -757,7 +757,7 @@ k_xpass3(const XDesc<T> *__restrict__ descs, Roots<T> rt, int ncol, int hz) {
         for (int it = 0; it < ITX; ++it)
             if ((TOTX % 64 == 0) || lane + it * 64 < TOTX)
 #pragma unroll
-                for (int r = 0; r < RX; ++r) gstore(d.Sstore + sbase + (unsigned)((it * RX + r) * 64 + lane), v[it][r]);
+                for (int r = 0; r < RX; ++r) gstore_nt(d.Sstore + sbase + (unsigned)((it * RX + r) * 64 + lane), v[it][r]);
     }
     double acc[3] = {0.0, 0.0, 0.0};
 #pragma unroll
@@ -772,7 +772,7 @@ k_xpass3(const XDesc<T> *__restrict__ descs, Roots<T> rt, int ncol, int hz) {
                                   : gload(d.fx + x) * gload(d.fy + c / hz) * gload(d.fz + c % hz);
                 if (ADJ == 2 && d.Sdot) {
                     // sum_k w(kz) Re(conj(Y) X) K dK/dl_d / K, X = this chain, Y = partner
-                    const cx<T> y = gload(d.Sdot + sbase + (unsigned)((it * RX + r) * 64 + lane));
+                    const cx<T> y = gload_nt(d.Sdot + sbase + (unsigned)((it * RX + r) * 64 + lane));
                     const int ky = c / hz, kz = c % hz;
                     const double wz = (kz == 0 || kz == hz - 1) ? 1.0 : 2.0;
                     const double wk = wz * (double)f * ((double)y.x * (double)v[it][r].x + (double)y.y * (double)v[it][r].y);
@@ -862,13 +862,13 @@ k_xpass4(const XDesc<T> *__restrict__ descs, Roots<T> rt, int ncol, int hz, int 
                 cx<T> lo = add<T>(ve[it][r], to), hi = sub<T>(ve[it][r], to);
                 const unsigned so = sbase + (unsigned)((it * RX + r) * 64 + lane);
                 if (ADJ >= 1 && d.Sstore) {
-                    gstore(d.Sstore + so, lo);
-                    gstore(d.Sstore + so + ITX * RX * 64, hi);
+                    gstore_nt(d.Sstore + so, lo);
+                    gstore_nt(d.Sstore + so + ITX * RX * 64, hi);
                 }
                 const T fx = gload(d.fx + x);
                 const T f0 = fx * fy0 * fz, f1 = fx * fy1 * fz;
                 if (ADJ == 2 && d.Sdot) {
-                    const cx<T> y0 = gload(d.Sdot + so), y1 = gload(d.Sdot + so + ITX * RX * 64);
+                    const cx<T> y0 = gload_nt(d.Sdot + so), y1 = gload_nt(d.Sdot + so + ITX * RX * 64);
                     const double p0 = wz * (double)f0 * ((double)y0.x * (double)lo.x + (double)y0.y * (double)lo.y);
                     const double p1 = wz * (double)f1 * ((double)y1.x * (double)hi.x + (double)y1.y * (double)hi.y);
                     const double gx = gload(d.gx + x);

@@ -97,6 +97,49 @@ def run_form(precision, steps, warmup, adjoint, L=128, Ns=200, algo="RQM4"):
     return out, first
 
 
+def run_batch(precision, nsys, steps, warmup, adjoint, L=128, Ns=200):
+    """nsys independent copies of the 128^3 system (replica noise streams)
+    updated as one batch: every propagator launch carries the chains of all
+    of them.  Returns the whole batch's contour-steps/s and the per-system
+    lockstep step time of the shared program."""
+    from polystring_amd import capi, workloads
+    ctx = capi.Context(0, precision)
+    models = []
+    for r in range(nsys):
+        wl = workloads.cfg2(L, Ns, replica=r)
+        models.append(capi.ModelAB(ctx, wl.wA, wl.wB, **wl.model_kwargs()))
+    b = capi.Batch(models, adjoint=adjoint)
+    shape = b.launch_shape()
+    b.update(warmup)
+    ctx.synchronize()
+    b.set_profiling(True)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    b.update(steps)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    prof = b.profile()
+    b.set_profiling(False)
+    obs = [m.observe() for m in models]
+    stress = [m.last_stress().tolist() for m in models] if adjoint else None
+    p = 8 if precision == "fp64" else 4
+    per_sys = 2 * workloads.algorithmic_bytes_per_step(wl.shape, p)
+    if adjoint:
+        per_sys += workloads.adjoint_bytes_per_lockstep_step(wl.shape, p)
+    n_launch = prof["iters"] * prof["lockstep_steps"]
+    group_ms = prof["prop_ms"] / n_launch  # one lockstep step of every system
+    out = dict(precision=precision, adjoint=adjoint, systems=nsys, wall_s=wall, iters=steps,
+               contour_steps_per_s=b.contour_steps() * steps / wall, iters_per_s=nsys * steps / wall,
+               propagator_ms=prof["prop_ms"], density_anderson_ms=prof["rest_ms"],
+               lockstep_group_us=group_ms * 1e3, lockstep_step_us_per_system=group_ms * 1e3 / nsys,
+               bytes_per_launch=per_sys * nsys, achieved_GBs=per_sys * nsys / (group_ms * 1e-3) / 1e9,
+               H=[o["H"] for o in obs], residual=[o["residual"] for o in obs], stress=stress, launch_shape=shape)
+    b.close()
+    del models
+    ctx.close()
+    return out
+
+
 def run_string(iters=2, precision="fp32"):
     """The batched string method on the 32-image 64^3 string (f=0.33,
     chiN=17.5, Ns=150): per string iteration every image takes 10 EM updates,

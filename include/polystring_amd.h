@@ -250,6 +250,29 @@ int po_model_profile(po_model *m, double *out4);
  * PO_DUAL variables pin. */
 int po_model_launch_shape(po_model *m, int *out5);
 
+/* ----------------------------------------------------------------- batch */
+/* Batched solves; no counterpart in the reference, which solves one system
+ * per process.  n models of one context, one grid, one algorithm (RQM4 or
+ * OS2) and one contour length share a single propagator launch program:
+ * every launch advances the chains of all n systems (the forward chains on
+ * the context's stream, the complementary chains on a second one).  Each
+ * model keeps its own fields, Anderson history and observables, read with
+ * the po_model_* calls.  With adjoint != 0 every batched update also fuses
+ * the cell stress (po_model_last_stress of each model, as
+ * po_model_set_adjoint).  The models stay usable alone; a batch must be
+ * destroyed before its models. */
+typedef struct po_batch po_batch;
+int po_batch_create(po_model **models, int n, int adjoint, po_batch **out);
+int po_batch_destroy(po_batch *b);
+/* n_iter Model_AB::update() of every model, queued without host synchronisation */
+int po_batch_update(po_batch *b, int n_iter);
+/* contour steps of one batched update (2 Ns per system) */
+long po_batch_contour_steps(const po_batch *b);
+/* the po_model_set_profiling / po_model_profile hooks of the shared program */
+int po_batch_set_profiling(po_batch *b, int on);
+int po_batch_profile(po_batch *b, double *out4);
+int po_batch_launch_shape(po_batch *b, int *out5);
+
 #ifdef __cplusplus
 }
 #endif

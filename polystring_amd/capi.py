@@ -46,6 +46,8 @@ EXPORTS = [
     "po_model_set_fields", "po_string_create", "po_string_destroy", "po_string_iterate", "po_string_get",
     "po_string_contour_steps", "po_model_set_graph", "po_models_relax",
     "po_model_set_profiling", "po_model_profile", "po_model_launch_shape",
+    "po_batch_create", "po_batch_destroy", "po_batch_update", "po_batch_contour_steps",
+    "po_batch_set_profiling", "po_batch_profile", "po_batch_launch_shape",
 ]
 
 
@@ -114,6 +116,10 @@ def load():
         "po_models_relax": ([ctypes.POINTER(_vp), i, i, i, i, d, ctypes.POINTER(i), _dp, _dp], i),
         "po_model_set_profiling": ([_vp, i], i), "po_model_profile": ([_vp, _dp], i),
         "po_model_launch_shape": ([_vp, ctypes.POINTER(ctypes.c_int)], i),
+        "po_batch_create": ([ctypes.POINTER(_vp), i, i, ctypes.POINTER(_vp)], i), "po_batch_destroy": ([_vp], i),
+        "po_batch_update": ([_vp, i], i), "po_batch_contour_steps": ([_vp], ctypes.c_long),
+        "po_batch_set_profiling": ([_vp, i], i), "po_batch_profile": ([_vp, _dp], i),
+        "po_batch_launch_shape": ([_vp, ctypes.POINTER(ctypes.c_int)], i),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -342,6 +348,49 @@ def relax_models(models, max_iter, min_iter=10, record_interval=1, thresh=1e-8):
     _check(load().po_models_relax(hs, n, int(max_iter), int(min_iter), int(record_interval), float(thresh), it,
                                   _p(H), _p(r)))
     return np.array(list(it)), H, r
+
+
+class Batch:
+    """Independent ModelAB systems of one grid and contour length updated
+    together: one shared propagator launch program advances the chains of
+    all of them (po_batch_*).  The models keep their own state; read it with
+    their own observe / get / last_stress.  adjoint=True fuses the cell
+    stress into every batched update."""
+
+    def __init__(self, models, adjoint=False):
+        self.models = list(models)  # keeps the models alive while the batch exists
+        n = len(self.models)
+        hs = (_vp * n)(*[m.h for m in self.models])
+        h = _vp()
+        _check(load().po_batch_create(hs, n, 1 if adjoint else 0, ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().po_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def update(self, n=1):
+        _check(load().po_batch_update(self.h, int(n)))
+
+    def contour_steps(self):
+        return load().po_batch_contour_steps(self.h)
+
+    def set_profiling(self, on=True):
+        _check(load().po_batch_set_profiling(self.h, 1 if on else 0))
+
+    def profile(self):
+        o = np.zeros(4)
+        _check(load().po_batch_profile(self.h, _p(o)))
+        return dict(prop_ms=o[0], rest_ms=o[1], iters=int(o[2]), lockstep_steps=int(o[3]))
+
+    def launch_shape(self):
+        o = (ctypes.c_int * 5)()
+        _check(load().po_batch_launch_shape(self.h, o))
+        return dict(PO_SPLIT=o[0], PO_PLANE_VARIANT=o[1], PO_XPASS_VARIANT=o[2], PO_PLANE_HOLD=o[3], PO_DUAL=o[4])
 
 
 class String:

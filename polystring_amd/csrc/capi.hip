@@ -336,7 +336,11 @@ template <typename T> struct Model : po_model {
 
     int update(int iters) override {
         hipStream_t st = ctx->stream;
-        const long M = sh.M;
+        if (fused_on && prog_adj.seq.empty()) {
+            fused_on = false;
+            int rc = set_adjoint(true);
+            if (rc) return rc;
+        }
         for (int it = 0; it < iters; ++it) {
             PO_HIP(mark());
             Program<T> &pg = fused_on ? prog_adj : prog;
@@ -352,46 +356,56 @@ template <typename T> struct Model : po_model {
                 rc = pg.run_graph(sh, rt, st);
             }
             if (rc) return rc;
-            if (fused_on) {
-                PO_HIP(launch_sum_rows(fusedPart.p,
-                                       (long)prog_adj.ndot * xpass_blocks(sh, prog_adj.x_variant, prog_adj.split), 3,
-                                       part.p, fusedSum.p, st));
-                fused_valid = true;
-            } else {
-                fused_valid = false;
-            }
             PO_HIP(mark());
-            // Q = qB->Qt() (src/Model_AB.cc:195)
-            PO_HIP(launch_mean_partials<T>(qB.p + (size_t)(sB - 1) * M, M, part.p, st));
-            PO_HIP(launch_finish_partials(part.p, kRedBlocks, 1, (double)M, obs.p + OBS_Q, st));
-            PO_HIP(launch_density<T>(qA.p, qAc.p, sA, M, quadA, obs.p + OBS_Q, 0.0, dsA, phiA.p, st));
-            PO_HIP(launch_density<T>(qB.p, qBc.p, sB, M, quadB, obs.p + OBS_Q, 0.0, dsB, phiB.p, st));
-            AndersonArgs a{};
-            a.wA = wAh.p;
-            a.wB = wBh.p;
-            a.dA = dAh.p;
-            a.dB = dBh.p;
-            a.M = M;
-            a.n = n;
-            a.nn = k < n ? (int)k : n;
-            a.nc = a.nn;
-            a.cur = cur;
-            a.next = cur > 0 ? cur - 1 : n;
-            for (int j = 0; j <= a.nn; ++j) a.hidx[j] = find_index(j);
-            a.chiN = desc.chiN;
-            a.lamA = 1.0 - std::pow(desc.lamA, (double)(k + 1));
-            a.lamB = 1.0 - std::pow(desc.lamB, (double)(k + 1));
-            PO_HIP(launch_anderson_d(a, phiA.p, phiB.p, wA.p, wB.p, part.p, st));
-            PO_HIP(launch_anderson_solve(a, part.p, BA.p, BB.p, C.p, obs.p, st));
-            FactorOut<T> fo{eA2.p, eA4.p, eB2.p, eB4.p, -0.5 * dsA, -0.25 * dsA, -0.5 * dsB, -0.25 * dsB, 0};
-            if (desc.algo == PO_ETDRK4) fo = FactorOut<T>{eA2.p, nullptr, eB2.p, nullptr, -1.0, 0.0, -1.0, 0.0, 1};
-            PO_HIP(launch_anderson_mix<T>(a, C.p, wA.p, wB.p, phiA.p, phiB.p, fo, part.p, st));
-            PO_HIP(launch_observables(part.p, M, obs.p, st));
-            PO_HIP(mark());
-            if (profiling) ++prof_iters;
-            cur = a.next;
-            ++k;
+            if ((rc = post_update(fused_on ? (long)prog_adj.ndot * xpass_blocks(sh, prog_adj.x_variant, prog_adj.split)
+                                           : 0)))
+                return rc;
         }
+        return PO_OK;
+    }
+
+    // Everything of one Model_AB::update after the propagator chains: the
+    // fused stress sum (rows > 0: partial rows of the fused adjoint), Q, the
+    // densities and the Anderson step, enqueued on the context's stream.
+    int post_update(long fused_rows) {
+        hipStream_t st = ctx->stream;
+        const long M = sh.M;
+        if (fused_rows > 0) {
+            PO_HIP(launch_sum_rows(fusedPart.p, fused_rows, 3, part.p, fusedSum.p, st));
+            fused_valid = true;
+        } else {
+            fused_valid = false;
+        }
+        // Q = qB->Qt() (src/Model_AB.cc:195)
+        PO_HIP(launch_mean_partials<T>(qB.p + (size_t)(sB - 1) * M, M, part.p, st));
+        PO_HIP(launch_finish_partials(part.p, kRedBlocks, 1, (double)M, obs.p + OBS_Q, st));
+        PO_HIP(launch_density<T>(qA.p, qAc.p, sA, M, quadA, obs.p + OBS_Q, 0.0, dsA, phiA.p, st));
+        PO_HIP(launch_density<T>(qB.p, qBc.p, sB, M, quadB, obs.p + OBS_Q, 0.0, dsB, phiB.p, st));
+        AndersonArgs a{};
+        a.wA = wAh.p;
+        a.wB = wBh.p;
+        a.dA = dAh.p;
+        a.dB = dBh.p;
+        a.M = M;
+        a.n = n;
+        a.nn = k < n ? (int)k : n;
+        a.nc = a.nn;
+        a.cur = cur;
+        a.next = cur > 0 ? cur - 1 : n;
+        for (int j = 0; j <= a.nn; ++j) a.hidx[j] = find_index(j);
+        a.chiN = desc.chiN;
+        a.lamA = 1.0 - std::pow(desc.lamA, (double)(k + 1));
+        a.lamB = 1.0 - std::pow(desc.lamB, (double)(k + 1));
+        PO_HIP(launch_anderson_d(a, phiA.p, phiB.p, wA.p, wB.p, part.p, st));
+        PO_HIP(launch_anderson_solve(a, part.p, BA.p, BB.p, C.p, obs.p, st));
+        FactorOut<T> fo{eA2.p, eA4.p, eB2.p, eB4.p, -0.5 * dsA, -0.25 * dsA, -0.5 * dsB, -0.25 * dsB, 0};
+        if (desc.algo == PO_ETDRK4) fo = FactorOut<T>{eA2.p, nullptr, eB2.p, nullptr, -1.0, 0.0, -1.0, 0.0, 1};
+        PO_HIP(launch_anderson_mix<T>(a, C.p, wA.p, wB.p, phiA.p, phiB.p, fo, part.p, st));
+        PO_HIP(launch_observables(part.p, M, obs.p, st));
+        PO_HIP(mark());
+        if (profiling) ++prof_iters;
+        cur = a.next;
+        ++k;
         return PO_OK;
     }
 
@@ -434,30 +448,18 @@ template <typename T> struct Model : po_model {
             fused_on = true;
             return PO_OK;
         }
-        const bool rqm4 = desc.algo == PO_RQM4;
         // The fused program keeps the plain program's tuned plane shape but
         // uses the 16-column x tiles unless PO_XPASS_VARIANT pins the tile:
         // with the adjoint dot compiled in, the 32-column kernel needs far more
         // registers (FP32 234 vs 150 VGPRs), which the plain program's
         // timing does not see.
         const int xv = std::getenv("PO_XPASS_VARIANT") ? prog.x_variant : 0;
-        const long elems = xstore_elems<T>(sh, xv, prog.split);
-        const int N = (sA - 1) + (sB - 1);
-        if (elems < 0 || N % 2 != 0 || (!rqm4 && !cell.ortho))
-            return fail(PO_ERR_INVALID, "the fused adjoint needs Ly*(Lz/2+1) divisible by 16, an even number of "
-                                        "contour steps and separable k tables; po_model_stress works for every grid");
         hipStream_t st = ctx->stream;
-        const int nk = rqm4 ? 3 : 1;
         AdjointArena<T> a;
-        a.elems = elems;
-        a.nblk = xpass_blocks(sh, xv, prog.split);
-        PO_HIP(fusedSpec.alloc((size_t)(N / 2) * 2 * nk * elems));
-        PO_HIP(fusedPart.alloc((size_t)(N / 2) * 2 * nk * a.nblk * 3));
-        PO_HIP(fusedSum.alloc(3));
-        a.spec = fusedSpec.p;
-        a.partials = fusedPart.p;
+        int rc = fused_arena(xv, prog.split, a);
+        if (rc) return rc;
         prog_adj.slice_elems = sh.M;
-        int rc = prog_adj.build(desc.algo, lanes_, st, &a);
+        rc = prog_adj.build(desc.algo, lanes_, st, &a);
         if (rc) {
             prog_adj.seq.clear();
             return rc;
@@ -467,6 +469,33 @@ template <typename T> struct Model : po_model {
         prog_adj.split = prog.split;
         prog_adj.hold = prog.hold;
         fused_on = true;
+        return PO_OK;
+    }
+
+    // The fused adjoint's arena for an x-pencil tile (xv) and spectrum layout
+    // (split): (Ns/2) steps x 2 chains x kinds stored spectra, one partial
+    // row block per dot descriptor and the 3-component sum.  Reallocated
+    // only when the layout asks for a different size.
+    int fused_arena(int xv, int split, AdjointArena<T> &a) {
+        const bool rqm4 = desc.algo == PO_RQM4;
+        if (desc.algo == PO_ETDRK4) return fail(PO_ERR_INVALID, "ETDRK4 has no adjoint");
+        const long elems = xstore_elems<T>(sh, xv, split);
+        const int N = (sA - 1) + (sB - 1);
+        if (elems < 0 || N % 2 != 0 || (!rqm4 && !cell.ortho))
+            return fail(PO_ERR_INVALID, "the fused adjoint needs Ly*(Lz/2+1) divisible by 16, an even number of "
+                                        "contour steps and separable k tables; po_model_stress works for every grid");
+        const int nk = rqm4 ? 3 : 1;
+        a.elems = elems;
+        a.nblk = xpass_blocks(sh, xv, split);
+        const size_t ns = (size_t)(N / 2) * 2 * nk * elems, np = (size_t)(N / 2) * 2 * nk * a.nblk * 3;
+        // the model's own fused program points into these buffers: drop it,
+        // update() rebuilds it on its next fused run
+        if (fusedSpec.n != ns || fusedPart.n != np) prog_adj.reset();
+        if (fusedSpec.n != ns) PO_HIP(fusedSpec.alloc(ns));
+        if (fusedPart.n != np) PO_HIP(fusedPart.alloc(np));
+        if (fusedSum.n != 3) PO_HIP(fusedSum.alloc(3));
+        a.spec = fusedSpec.p;
+        a.partials = fusedPart.p;
         return PO_OK;
     }
 
@@ -862,6 +891,138 @@ template <typename T> struct Model : po_model {
         PO_HIP(hipMemcpyAsync(out, src, sizeof(double) * sh.M, hipMemcpyDeviceToHost, ctx->stream));
         PO_HIP(hipStreamSynchronize(ctx->stream));
         return PO_OK;
+    }
+};
+
+}  // namespace po
+
+// ------------------------------------------------------------------ batch --
+// Independent Model_AB systems of one grid and contour length advanced as
+// lanes of one launch program: every launch carries the chains of all of
+// them, so a 128^3 launch fills the 256 CUs several times over instead of
+// once.  Each model keeps its own fields, slices, Anderson history and
+// observables; only the propagator program is shared.
+struct po_batch {
+    po_ctx *ctx = nullptr;
+    bool profiling = false;
+    std::vector<hipEvent_t> ev;
+    size_t ev_used = 0;
+    int prof_iters = 0;
+    long steps = 0;     // contour steps of one update of every system
+    long lockstep = 0;  // lockstep launch groups per update (Ns)
+    virtual ~po_batch() {
+        for (auto e : ev) (void)hipEventDestroy(e);
+    }
+    hipError_t mark() {
+        if (!profiling) return hipSuccess;
+        if (ev_used == ev.size()) {
+            hipEvent_t e;
+            hipError_t r = hipEventCreate(&e);
+            if (r != hipSuccess) return r;
+            ev.push_back(e);
+        }
+        return hipEventRecord(ev[ev_used++], ctx->stream);
+    }
+    virtual int update(int n) = 0;
+    virtual void launch_shape(int *out5) = 0;
+};
+
+namespace po {
+
+template <typename T> struct Batch : po_batch {
+    std::vector<Model<T> *> ms;
+    Program<T> prog;
+    bool adjoint = false;
+    hipStream_t st2 = nullptr;
+    hipEvent_t dev[4] = {};
+
+    int init(po_model **list, int n, bool adj) {
+        for (int i = 0; i < n; ++i) {
+            auto *m = dynamic_cast<Model<T> *>(list[i]);
+            if (!m) return fail(PO_ERR_INVALID, "batch models must share one precision");
+            ms.push_back(m);
+        }
+        Model<T> &m0 = *ms[0];
+        ctx = m0.ctx;
+        adjoint = adj;
+        if (m0.desc.algo == PO_ETDRK4) return fail(PO_ERR_INVALID, "a batch runs RQM4 or OS2 programs");
+        for (auto *m : ms) {
+            if (m->ctx != ctx) return fail(PO_ERR_INVALID, "batch models must share one context");
+            if (m->sh.Lx != m0.sh.Lx || m->sh.Ly != m0.sh.Ly || m->sh.Lz != m0.sh.Lz ||
+                m->desc.grid.dim != m0.desc.grid.dim)
+                return fail(PO_ERR_INVALID, "batch models must share one grid");
+            if (m->desc.algo != m0.desc.algo) return fail(PO_ERR_INVALID, "batch models must share one algorithm");
+            if (m->steps != m0.steps) return fail(PO_ERR_INVALID, "batch models must share one contour length");
+        }
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < i; ++j)
+                if (ms[i] == ms[j]) return fail(PO_ERR_INVALID, "a model appears twice in the batch");
+        // the first model's tuned shape; the fused adjoint takes the
+        // 16-column x tiles unless PO_XPASS_VARIANT pins the tile (the same
+        // rule as a single model's fused program)
+        const int split = m0.prog.split;
+        const int xv = (adj && !std::getenv("PO_XPASS_VARIANT")) ? 0 : m0.prog.x_variant;
+        std::vector<Lane<T>> lanes;
+        for (int role = 0; role < 2; ++role)
+            for (auto *m : ms) lanes.push_back(m->lanes_[role]);
+        std::vector<AdjointArena<T>> arenas(adj ? n : 0);
+        int rc;
+        for (int i = 0; i < (adj ? n : 0); ++i)
+            if ((rc = ms[i]->fused_arena(xv, split, arenas[i]))) return rc;
+        prog.slice_elems = m0.sh.M;
+        if ((rc = prog.build(m0.desc.algo, lanes, ctx->stream, adj ? arenas.data() : nullptr, n))) return rc;
+        prog.plane_variant = m0.prog.plane_variant;
+        prog.x_variant = xv;
+        prog.split = split;
+        prog.hold = m0.prog.hold;
+        steps = m0.steps * n;
+        lockstep = m0.steps / 2;
+        bool ok = hipStreamCreateWithFlags(&st2, hipStreamNonBlocking) == hipSuccess;
+        for (auto &e : dev) ok = ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
+        if (!ok) return fail(PO_ERR_DEVICE, "batch stream / event creation failed");
+        return PO_OK;
+    }
+
+    // One Model_AB::update of every system: the shared program with the
+    // forward chains on the context's stream and the complementary chains on
+    // the second one, then each model's stress sum, Q, densities and
+    // Anderson step on the context's stream.
+    int update(int iters) override {
+        hipStream_t st = ctx->stream;
+        Model<T> &m0 = *ms[0];
+        const long rows = adjoint ? (long)prog.ndot * xpass_blocks(m0.sh, prog.x_variant, prog.split) : 0;
+        for (int it = 0; it < iters; ++it) {
+            PO_HIP(mark());
+            PO_HIP(hipEventRecord(dev[0], st));
+            PO_HIP(hipStreamWaitEvent(st2, dev[0], 0));
+            int rc = prog.run_dual(m0.sh, m0.rt, st, st2, dev[1], dev[2]);
+            if (rc) return rc;
+            PO_HIP(hipEventRecord(dev[3], st2));
+            PO_HIP(hipStreamWaitEvent(st, dev[3], 0));
+            PO_HIP(mark());
+            for (auto *m : ms)
+                if ((rc = m->post_update(rows))) return rc;
+            PO_HIP(mark());
+            if (profiling) ++prof_iters;
+        }
+        return PO_OK;
+    }
+
+    void launch_shape(int *o) override {
+        o[0] = prog.split;
+        o[1] = prog.plane_variant;
+        o[2] = prog.x_variant;
+        o[3] = prog.hold;
+        o[4] = 1;
+    }
+
+    ~Batch() override {
+        if (st2) {
+            (void)hipStreamSynchronize(st2);
+            (void)hipStreamDestroy(st2);
+        }
+        for (auto e : dev)
+            if (e) (void)hipEventDestroy(e);
     }
 };
 
@@ -1280,6 +1441,80 @@ int po_model_profile(po_model *m, double *out4) {
     out4[1] = rest;
     out4[2] = m->prof_iters;
     out4[3] = (double)(m->steps / 2);  // lockstep launches of the two chains per update
+    return PO_OK;
+}
+
+int po_batch_create(po_model **models, int n, int adjoint, po_batch **out) {
+    if (!models || n < 1 || !out) return fail(PO_ERR_INVALID, "bad argument");
+    for (int i = 0; i < n; ++i)
+        if (!models[i]) return fail(PO_ERR_INVALID, "null model in the list");
+    PO_HIP(hipSetDevice(models[0]->ctx->device));
+    PO_HIP(hipStreamSynchronize(models[0]->ctx->stream));
+    po_batch *b;
+    int rc;
+    if (models[0]->ctx->precision == PO_FP64) {
+        auto *t = new po::Batch<double>();
+        rc = t->init(models, n, adjoint != 0);
+        b = t;
+    } else {
+        auto *t = new po::Batch<float>();
+        rc = t->init(models, n, adjoint != 0);
+        b = t;
+    }
+    if (rc) {
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return PO_OK;
+}
+
+int po_batch_destroy(po_batch *b) {
+    if (!b) return PO_OK;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+    delete b;
+    return PO_OK;
+}
+
+int po_batch_update(po_batch *b, int n_iter) {
+    if (!b || n_iter < 0) return fail(PO_ERR_INVALID, "bad argument");
+    PO_HIP(hipSetDevice(b->ctx->device));
+    return b->update(n_iter);
+}
+
+long po_batch_contour_steps(const po_batch *b) { return b ? b->steps : -1; }
+
+int po_batch_set_profiling(po_batch *b, int on) {
+    if (!b) return fail(PO_ERR_INVALID, "null batch");
+    PO_HIP(hipStreamSynchronize(b->ctx->stream));
+    b->profiling = on != 0;
+    b->ev_used = 0;
+    b->prof_iters = 0;
+    return PO_OK;
+}
+
+int po_batch_profile(po_batch *b, double *out4) {
+    if (!b || !out4) return fail(PO_ERR_INVALID, "null argument");
+    PO_HIP(hipStreamSynchronize(b->ctx->stream));
+    double prop = 0.0, rest = 0.0;
+    for (int i = 0; i < b->prof_iters; ++i) {
+        float x = 0.f, y = 0.f;
+        PO_HIP(hipEventElapsedTime(&x, b->ev[3 * i], b->ev[3 * i + 1]));
+        PO_HIP(hipEventElapsedTime(&y, b->ev[3 * i + 1], b->ev[3 * i + 2]));
+        prop += x;
+        rest += y;
+    }
+    out4[0] = prop;
+    out4[1] = rest;
+    out4[2] = b->prof_iters;
+    out4[3] = (double)b->lockstep;
+    return PO_OK;
+}
+
+int po_batch_launch_shape(po_batch *b, int *out5) {
+    if (!b || !out5) return fail(PO_ERR_INVALID, "null argument");
+    b->launch_shape(out5);
     return PO_OK;
 }
 

@@ -744,14 +744,26 @@ k_xpass3(const XDesc<T> *__restrict__ descs, Roots<T> rt, int ncol, int hz) {
     const auto xmap = [w](int p, int e, int, int) { return X::loc(e, w * CPW + p); };
     cx<T> *lds = buf;
     cx<T> v[ITX][RX];
+    // full 3-D spectrum of this wave's columns, in register order (stored
+    // for, or read from, the partner chain of the adjoint stress)
+    const unsigned sbase = (blockIdx.x * X::NW + w) * (unsigned)(ITX * RX * 64);
+    // FP32: the partner's spectrum is loaded before the forward transform
+    // (latency overlapped with it, as in k_xpass4)
+    constexpr bool PF = ADJ == 2 && sizeof(T) == 4;
+    cx<T> yd[PF ? ITX : 1][PF ? RX : 1];
+    if (PF && d.Sdot) {
+#pragma unroll
+        for (int it = 0; it < ITX; ++it)
+            if ((TOTX % 64 == 0) || lane + it * 64 < TOTX)
+#pragma unroll
+                for (int r = 0; r < RX; ++r)
+                    yd[PF ? it : 0][PF ? r : 0] = gload_nt(d.Sdot + sbase + (unsigned)((it * RX + r) * 64 + lane));
+    }
     {
         cx<T> a[F::ITF][F::RF];
         WaveFFT<T, LX, false, CPW, true, IO_GLB, IO_REG>::run(lds, 1, RS, rt.x, lane, d.S + cw, 1, ncol, nullptr, a,
                                                               v, xmap);
     }
-    // full 3-D spectrum of this wave's columns, in register order (stored
-    // for, or read from, the partner chain of the adjoint stress)
-    const unsigned sbase = (blockIdx.x * X::NW + w) * (unsigned)(ITX * RX * 64);
     if (ADJ >= 1 && d.Sstore) {
 #pragma unroll
         for (int it = 0; it < ITX; ++it)
@@ -772,7 +784,8 @@ k_xpass3(const XDesc<T> *__restrict__ descs, Roots<T> rt, int ncol, int hz) {
                                   : gload(d.fx + x) * gload(d.fy + c / hz) * gload(d.fz + c % hz);
                 if (ADJ == 2 && d.Sdot) {
                     // sum_k w(kz) Re(conj(Y) X) K dK/dl_d / K, X = this chain, Y = partner
-                    const cx<T> y = gload_nt(d.Sdot + sbase + (unsigned)((it * RX + r) * 64 + lane));
+                    const cx<T> y = PF ? yd[PF ? it : 0][PF ? r : 0]
+                                       : gload_nt(d.Sdot + sbase + (unsigned)((it * RX + r) * 64 + lane));
                     const int ky = c / hz, kz = c % hz;
                     const double wz = (kz == 0 || kz == hz - 1) ? 1.0 : 2.0;
                     const double wk = wz * (double)f * ((double)y.x * (double)v[it][r].x + (double)y.y * (double)v[it][r].y);
@@ -833,6 +846,24 @@ k_xpass4(const XDesc<T> *__restrict__ descs, Roots<T> rt, int ncol, int hz, int 
     const auto xmap = [w](int p, int e, int, int) { return X::loc(e, w * CPW + p); };
     cx<T> *lds = buf;
     cx<T> ve[ITX][RX], vo[ITX][RX];
+    // stored spectra: the lo set (ky = k') then the hi set (ky = k' + Ly/2) of this wave
+    const unsigned sbase = (blockIdx.x * X::NW + w) * (unsigned)(2 * ITX * RX * 64);
+    // FP32: the partner's stored spectrum of the adjoint dot is loaded before
+    // the forward transforms, so its latency overlaps them instead of
+    // stalling the wave after them (FP64 has no registers to spare for it)
+    constexpr bool PF = ADJ == 2 && sizeof(T) == 4;
+    cx<T> yd[PF ? 2 : 1][PF ? ITX : 1][PF ? RX : 1];
+    if (PF && d.Sdot) {
+#pragma unroll
+        for (int it = 0; it < ITX; ++it)
+            if ((TOTX % 64 == 0) || lane + it * 64 < TOTX)
+#pragma unroll
+                for (int r = 0; r < RX; ++r) {
+                    const unsigned so = sbase + (unsigned)((it * RX + r) * 64 + lane);
+                    yd[0][PF ? it : 0][PF ? r : 0] = gload_nt(d.Sdot + so);
+                    yd[PF ? 1 : 0][PF ? it : 0][PF ? r : 0] = gload_nt(d.Sdot + so + ITX * RX * 64);
+                }
+    }
     {
         cx<T> a[F::ITF][F::RF];
         WaveFFT<T, LX, false, CPW, true, IO_GLB, IO_REG>::run(lds, 1, RS, rt.x, lane, d.S + cw, 1, ncol, nullptr, a,
@@ -843,8 +874,6 @@ k_xpass4(const XDesc<T> *__restrict__ descs, Roots<T> rt, int ncol, int hz, int 
         WaveFFT<T, LX, false, CPW, true, IO_GLB, IO_REG>::run(lds, 1, RS, rt.x, lane, d.S + half + cw, 1, ncol,
                                                               nullptr, a, vo, xmap);
     }
-    // stored spectra: the lo set (ky = k') then the hi set (ky = k' + Ly/2) of this wave
-    const unsigned sbase = (blockIdx.x * X::NW + w) * (unsigned)(2 * ITX * RX * 64);
     double acc[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int it = 0; it < ITX; ++it) {
@@ -868,7 +897,9 @@ k_xpass4(const XDesc<T> *__restrict__ descs, Roots<T> rt, int ncol, int hz, int 
                 const T fx = gload(d.fx + x);
                 const T f0 = fx * fy0 * fz, f1 = fx * fy1 * fz;
                 if (ADJ == 2 && d.Sdot) {
-                    const cx<T> y0 = gload_nt(d.Sdot + so), y1 = gload_nt(d.Sdot + so + ITX * RX * 64);
+                    const cx<T> y0 = PF ? yd[0][PF ? it : 0][PF ? r : 0] : gload_nt(d.Sdot + so);
+                    const cx<T> y1 = PF ? yd[PF ? 1 : 0][PF ? it : 0][PF ? r : 0]
+                                        : gload_nt(d.Sdot + so + ITX * RX * 64);
                     const double p0 = wz * (double)f0 * ((double)y0.x * (double)lo.x + (double)y0.y * (double)lo.y);
                     const double p1 = wz * (double)f1 * ((double)y1.x * (double)hi.x + (double)y1.y * (double)hi.y);
                     const double gx = gload(d.gx + x);

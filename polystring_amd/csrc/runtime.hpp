@@ -391,13 +391,18 @@ template <typename T> struct Program {
     DBuf<XDesc<T>> xd;
     DBuf<EtdDesc<T>> ed;
     std::vector<Launch> seq;
-    int ndot = 0;  // dot descriptors of a fused adjoint program
+    int ndot = 0;  // dot descriptors per chain pair of a fused adjoint program
     bool etd = false;  // an ETDRK4 program (unsplit layout, 16-column x tiles)
     // first launch of the second half of the contour in a fused adjoint
     // program: its x-passes read the partner lane's first-half spectra
     int mid_launch = -1;
 
-    int build(int algo, const std::vector<Lane<T>> &lanes, hipStream_t st, const AdjointArena<T> *adj = nullptr) {
+    // A fused adjoint program takes one arena per chain pair: with P pairs
+    // the lanes are the P forward chains followed by the P complementary
+    // chains (lane l is role l / P of pair l % P), so the two halves of every
+    // launch are the forward and the complementary chains.
+    int build(int algo, const std::vector<Lane<T>> &lanes, hipStream_t st, const AdjointArena<T> *adj = nullptr,
+              int npairs = 1) {
         if (algo == PO_ETDRK4) {
             if (adj) return fail(PO_ERR_INVALID, "ETDRK4 has no adjoint");
             return build_etd(lanes, st);
@@ -431,25 +436,28 @@ template <typename T> struct Program {
             X.insert(X.end(), v.begin(), v.end());
         };
         const bool rqm4 = algo == PO_RQM4;
-        if (adj && (L != 2 || nsteps % 2 != 0))
-            return fail(PO_ERR_INVALID, "the fused adjoint needs two lanes with an even step count");
+        if (adj && (npairs < 1 || L != 2 * npairs || nsteps % 2 != 0))
+            return fail(PO_ERR_INVALID, "the fused adjoint needs two lanes per chain pair with an even step count");
         ndot = 0;
+        std::vector<int> ndot_pair(adj ? npairs : 0, 0);
         // attach the adjoint store / dot of (step t, lane l, kind k) to an x-pass descriptor
         auto adjoin = [&](XDesc<T> x, const KTable<T> &kt, int t, int l, int k) {
             if (!adj) return x;
             const int half = nsteps / 2, nk = rqm4 ? 3 : 1;
+            const int pr = l % npairs, role = l / npairs;
+            const AdjointArena<T> &ar = adj[pr];
             x.gx = kt.gx.p;
             x.gy = kt.gy.p;
             x.gz = kt.gz.p;
             if (t < half) {
-                x.Sstore = adj->spec + ((size_t)(t * 2 + l) * nk + k) * adj->elems;
+                x.Sstore = ar.spec + ((size_t)(t * 2 + role) * nk + k) * ar.elems;
             } else {
                 static const int partner[3] = {0, 2, 1};
                 const int tp = nsteps - 1 - t;
-                x.Sdot = adj->spec + ((size_t)(tp * 2 + (1 - l)) * nk + partner[k]) * adj->elems;
+                x.Sdot = ar.spec + ((size_t)(tp * 2 + (1 - role)) * nk + partner[k]) * ar.elems;
                 x.coef = rqm4 ? (k == 0 ? -1.0 / 3.0 : 4.0 / 3.0) : 1.0;
-                x.partials = adj->partials + (size_t)ndot * adj->nblk * 3;
-                ++ndot;
+                x.partials = ar.partials + (size_t)ndot_pair[pr] * ar.nblk * 3;
+                ndot = ++ndot_pair[pr];  // dot descriptors per pair (equal for every pair)
             }
             return x;
         };
@@ -794,6 +802,14 @@ template <typename T> struct Program {
         }
         PO_HIP(hipGraphLaunch(exec, st));
         return PO_OK;
+    }
+
+    // Drops the launch sequence and any captured graph (descriptors that
+    // point into buffers about to be reallocated).
+    void reset() {
+        seq.clear();
+        if (exec) (void)hipGraphExecDestroy(exec);
+        exec = nullptr;
     }
 
     ~Program() {

@@ -79,13 +79,16 @@ def cfg1(L=32):
     return Workload("cfg1", shape, ("CUBIC", 4.0, 4.0, 4.0), 100, f, chiN, wA, wB)
 
 
-def cfg2(L=128, Ns=200):
-    """AB melt f=0.35, chiN=18, cubic a=4.2, 128^3, Ns=200, gyroid template."""
+def cfg2(L=128, Ns=200, replica=0):
+    """AB melt f=0.35, chiN=18, cubic a=4.2, 128^3, Ns=200, gyroid template.
+
+    replica r > 0 draws the noise from streams 2r (wA) and 2r + 1 (wB) of the
+    same seed: independent copies of the system for batched solves."""
     shape = (L, L, L)
     f, chiN, a = 0.35, 18.0, 4.2
     g = gyroid(shape, (a, a, a))
-    wA = chiN * (1 - f) + 2 * g + noise(1002, 0, shape, 0.2)
-    wB = chiN * f - 2 * g + noise(1002, 1, shape, 0.2)
+    wA = chiN * (1 - f) + 2 * g + noise(1002, 2 * replica, shape, 0.2)
+    wB = chiN * f - 2 * g + noise(1002, 2 * replica + 1, shape, 0.2)
     return Workload("cfg2", shape, ("CUBIC", a, a, a), Ns, f, chiN, wA, wB)
 
 

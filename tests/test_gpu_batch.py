@@ -1,0 +1,121 @@
+"""Batched solves (po_batch_*): independent Model_AB systems of one grid and
+contour length share one propagator launch program.
+
+The batch must give every model exactly what updating that model alone
+gives: the shared launches run the same kernels on the same per-lane
+descriptors, so with the launch variants pinned the results are
+bit-identical (H, Q, residual, fields, and the fused stress).  One model of
+each batch is also checked against the FP64 oracle to 1e-10.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from polystring_amd import capi
+
+pytestmark = pytest.mark.gpu
+
+PINS = ("PO_SPLIT", "PO_PLANE_VARIANT", "PO_XPASS_VARIANT", "PO_PLANE_HOLD")
+
+
+def smooth_field(shape, seed, amp=3.0, offset=1.0):
+    rng = np.random.default_rng(seed)
+    w = rng.standard_normal(shape)
+    W = np.fft.fftn(w)
+    k2 = sum(np.meshgrid(*[np.minimum(np.arange(n), n - np.arange(n)) ** 2 for n in shape], indexing="ij"))
+    W *= np.exp(-0.05 * k2)
+    w = np.real(np.fft.ifftn(W))
+    return amp * w / (np.abs(w).max() + 1e-300) + offset
+
+
+# three systems of one 32^3 grid and one contour length (Ns = 30) with
+# different cells, chiN and block lengths: per-lane k tables and segments
+SYSTEMS = [
+    dict(lengths=(3.8, 4.0, 4.3), Ms=(13, 19), chiN=15.0, seed=71),
+    dict(lengths=(4.1, 4.1, 4.1), Ms=(11, 21), chiN=18.0, seed=72),
+    dict(lengths=(3.6, 4.4, 3.9), Ms=(17, 15), chiN=12.5, seed=73),
+]
+
+
+def make(ctx, sys_, algo, shape=(32, 32, 32)):
+    wA = smooth_field(shape, sys_["seed"], offset=2.0)
+    wB = smooth_field(shape, sys_["seed"] + 100, offset=-1.0)
+    kw = dict(cell=("ORTHORHOMBIC",) + sys_["lengths"], Ms=sys_["Ms"], chiN=sys_["chiN"], algo=algo,
+              n_anderson=3)
+    return wA, wB, kw
+
+
+@pytest.mark.parametrize("split", [0, 1])
+@pytest.mark.parametrize("adjoint", [False, True])
+@pytest.mark.parametrize("algo", ["RQM4", "OS2"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_batch_matches_models_alone(ctx64, ctx32, precision, algo, adjoint, split, monkeypatch):
+    for k in PINS:
+        monkeypatch.setenv(k, "0")
+    monkeypatch.setenv("PO_SPLIT", str(split))
+    ctx = ctx64 if precision == "fp64" else ctx32
+    print(f"\n[batch] {precision} {algo} adjoint={adjoint} split={split}", flush=True)
+    alone, batched = [], []
+    for s in SYSTEMS:
+        wA, wB, kw = make(ctx, s, algo)
+        a = capi.ModelAB(ctx, wA, wB, **kw)
+        if adjoint:
+            a.set_adjoint(True)
+        alone.append(a)
+        batched.append(capi.ModelAB(ctx, wA, wB, **kw))
+    b = capi.Batch(batched, adjoint=adjoint)
+    assert b.launch_shape()["PO_SPLIT"] == split
+    assert b.contour_steps() == sum(m.contour_steps() for m in batched)
+    for it in range(3):
+        b.update()
+        for a, m in zip(alone, batched):
+            a.update()
+            oa, om = a.observe(), m.observe()
+            assert oa == om, (it, oa, om)
+            if adjoint:
+                assert np.array_equal(a.last_stress(), m.last_stress())
+    for a, m in zip(alone, batched):
+        for f in ("wA", "wB", "phiA", "phiB"):
+            assert np.array_equal(a.get(f), m.get(f)), f
+    b.close()
+    # the models stay usable on their own after the batch is gone
+    batched[0].update()
+    alone[0].update()
+    assert batched[0].observe() == alone[0].observe()
+
+
+def test_batch_member_matches_oracle(ctx64):
+    """The second member of an FP64 batch against the oracle: two SCFT
+    iterations to 1e-10 in H, Q and phi."""
+    models = []
+    for s in SYSTEMS:
+        wA, wB, kw = make(ctx64, s, "RQM4")
+        models.append(capi.ModelAB(ctx64, wA, wB, **kw))
+    b = capi.Batch(models, adjoint=True)
+    b.update(2)
+    s = SYSTEMS[1]
+    wA, wB, kw = make(ctx64, s, "RQM4")
+    kw.pop("algo")
+    om = O.Model(wA, wB, dim=3, mde="RQM4", **kw)
+    om.update()
+    om.update()
+    o = models[1].observe()
+    assert abs(o["H"] - om.H()) <= 1e-10 * abs(om.H())
+    assert np.abs(models[1].get("phiA") - om.get("phiA")).max() < 1e-10
+    assert np.abs(models[1].get("wB") - om.get("wB")).max() < 1e-10
+
+
+def test_batch_rejects_mismatched_models(ctx64, ctx32):
+    s = SYSTEMS[0]
+    wA, wB, kw = make(ctx64, s, "RQM4")
+    m0 = capi.ModelAB(ctx64, wA, wB, **kw)
+    wA2, wB2, kw2 = make(ctx64, s, "RQM4", shape=(16, 32, 32))
+    other_grid = capi.ModelAB(ctx64, wA2, wB2, **kw2)
+    other_len = capi.ModelAB(ctx64, wA, wB, **dict(kw, Ms=(13, 21)))
+    other_prec = capi.ModelAB(ctx32, wA, wB, **kw)
+    for bad in (other_grid, other_len, other_prec):
+        with pytest.raises(capi.POError) as e:
+            capi.Batch([m0, bad])
+        assert e.value.code == capi.PO_ERR_INVALID
+    with pytest.raises(capi.POError):
+        capi.Batch([m0, m0])

@@ -3,21 +3,26 @@
 Workload (the configuration the metric is quoted on): AB diblock melt,
 f_A = 0.35, chiN = 18, cubic cell a = 4.2, 128^3 grid, Ns = 200
 (Ms = [71, 131]), RQM4 propagator, Simpson quadrature, Anderson mixing
-(history 20, lam 0.1), seeded double-gyroid template + noise.  One "step" is
-one full SCFT iteration (Model_AB::update: four propagators, Q, densities,
-Anderson mixing) with every input resident in HBM; it advances 2*Ns = 400
-contour steps (adjoint work is not counted as contour steps).
+(history 20, lam 0.1), seeded double-gyroid template + noise.  One SCFT
+iteration (Model_AB::update: four propagators, Q, densities, Anderson mixing)
+advances 2*Ns = 400 contour steps (adjoint work is not counted); every input
+is resident in HBM.
 
-Two forms are timed in the same run:
-  * mixed + adjoint (the headline `value`): FP32 propagators with FP64 Q,
-    H, Gram sums and Anderson solve, and the cell-stress adjoint fused into
-    every update;
-  * FP64 plain: the same iteration in FP64 without the adjoint.
+Forms timed in the same run:
+  * mixed + adjoint, batched (the headline `value`): --systems independent
+    copies of the system (4 by default; replica noise streams, replica 0 is
+    the single-system fields) share one propagator launch program
+    (po_batch_*); FP32 propagators with FP64 Q, H, Gram sums and Anderson
+    solve, and the cell-stress adjoint fused into every update.  A "step" is
+    one batched update (one SCFT iteration of every system);
+  * mixed + adjoint, one system;
+  * FP64 plain, one system: the comparison the metric names;
+  * (extras) FP64 plain batched, the string method, the ragged scan, ETDRK4.
 The precision error is measured on the same fields: H at the initial fields
 (evaluated from the first update's phi and Q), ln Q, phi_A, phi_B and the
 residual of that update, FP32 against FP64.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--systems B]
 
 Prints one JSON line.
 """
@@ -110,7 +115,20 @@ def run_batch(precision, nsys, steps, warmup, adjoint, L=128, Ns=200):
         models.append(capi.ModelAB(ctx, wl.wA, wl.wB, **wl.model_kwargs()))
     b = capi.Batch(models, adjoint=adjoint)
     shape = b.launch_shape()
-    b.update(warmup)
+    print(f"[bench] batch of {nsys} launch shape {precision}: " + " ".join(f"{k}={v}" for k, v in shape.items()),
+          file=sys.stderr, flush=True)
+    # first batched update from the initial fields: replica 0 holds the
+    # single-system fields, so its H at those fields is the precision probe
+    import numpy as np
+    b.update(1)
+    m0 = models[0]
+    first = m0.observe()
+    phiA, phiB = m0.get("phiA"), m0.get("phiB")
+    wl0 = workloads.cfg2(L, Ns, replica=0)
+    chiN = wl0.model_kwargs()["chiN"]
+    first["H_fixed"] = float(np.mean(chiN * phiA * phiB - wl0.wA * phiA - wl0.wB * phiB) - np.log(first["Q"]))
+    if warmup > 1:
+        b.update(warmup - 1)
     ctx.synchronize()
     b.set_profiling(True)
     ctx.synchronize()
@@ -133,7 +151,8 @@ def run_batch(precision, nsys, steps, warmup, adjoint, L=128, Ns=200):
                propagator_ms=prof["prop_ms"], density_anderson_ms=prof["rest_ms"],
                lockstep_group_us=group_ms * 1e3, lockstep_step_us_per_system=group_ms * 1e3 / nsys,
                bytes_per_launch=per_sys * nsys, achieved_GBs=per_sys * nsys / (group_ms * 1e-3) / 1e9,
-               H=[o["H"] for o in obs], residual=[o["residual"] for o in obs], stress=stress, launch_shape=shape)
+               H=[o["H"] for o in obs], residual=[o["residual"] for o in obs], stress=stress, launch_shape=shape,
+               H_fixed_replica0=first["H_fixed"])
     b.close()
     del models
     ctx.close()
@@ -263,13 +282,14 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--copy-probe", action="store_true", help="run only the traffic-calibration copy")
     ap.add_argument("--copy-rate", action="store_true", help="print the measured device copy rate only")
-    ap.add_argument("--form", choices=["both", "mixed", "fp64"], default="both",
+    ap.add_argument("--form", choices=["all", "batch", "mixed", "fp64"], default="all",
                     help="time only one form (for profiling runs)")
+    ap.add_argument("--systems", type=int, default=4, help="systems per batch of the headline form")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-extras", action="store_true", help="skip the string and ragged-scan lines")
+    ap.add_argument("--no-extras", action="store_true", help="skip the string, ragged-scan and ETDRK4 lines")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
@@ -282,23 +302,35 @@ def main():
               "independent solves / string images); use --gpus 1", file=sys.stderr)
         sys.exit(2)
     warmup = max(1, args.warmup)
+    nsys = args.systems
 
-    mixed = f64 = first32 = first64 = None
-    if args.form in ("both", "mixed"):
+    batch = mixed = f64 = first32 = first64 = None
+    if args.form in ("all", "batch"):
+        # the headline: nsys systems per batched update; steps counts batched updates
+        batch = run_batch("fp32", nsys, args.steps, warmup, adjoint=True)
+        print(f"[bench] mixed+adjoint batched {json.dumps(batch)}", file=sys.stderr, flush=True)
+    if args.form in ("all", "mixed"):
         mixed, first32 = run_form("fp32", args.steps, warmup, adjoint=True)
-        print(f"[bench] mixed+adjoint {json.dumps(mixed)}", file=sys.stderr, flush=True)
-    if args.form in ("both", "fp64"):
+        print(f"[bench] mixed+adjoint single {json.dumps(mixed)}", file=sys.stderr, flush=True)
+    if args.form in ("all", "fp64"):
         f64, first64 = run_form("fp64", args.steps, warmup, adjoint=False)
         print(f"[bench] fp64 plain {json.dumps(f64)}", file=sys.stderr, flush=True)
-    head = mixed if mixed is not None else f64
+    head = batch if batch is not None else mixed if mixed is not None else f64
+    head_kind = "fp32_adjoint_batch%d" % nsys if head is batch else "fp32_adjoint" if head is mixed else "fp64"
 
     traffic = None
     if os.path.exists(PROFILE_TRAFFIC):
         with open(PROFILE_TRAFFIC) as fh:
-            key = "hbm_bytes_per_lockstep_step_" + ("fp32_adjoint" if head is mixed else "fp64")
-            traffic = json.load(fh).get(key)
+            traffic = json.load(fh).get("hbm_bytes_per_lockstep_step_" + head_kind)
     extras = {}
-    if not args.no_extras and args.form == "both":
+    if not args.no_extras and args.form == "all":
+        f64b = run_batch("fp64", nsys, max(1, args.steps // 5), 1, adjoint=False)
+        extras["fp64_plain_batched"] = {
+            "workload": f"the same {nsys} systems batched in FP64 without the adjoint",
+            "contour_steps_per_s": f64b["contour_steps_per_s"],
+            "lockstep_step_us_per_system": f64b["lockstep_step_us_per_system"],
+            "achieved_GBs": f64b["achieved_GBs"], "roofline_frac": f64b["achieved_GBs"] / HBM_PEAK_GBS}
+        print(f"[bench] fp64 batched {json.dumps(extras['fp64_plain_batched'])}", file=sys.stderr, flush=True)
         extras["string_batched"] = run_string()
         print(f"[bench] string {json.dumps(extras['string_batched'])}", file=sys.stderr, flush=True)
         extras["ragged_scan"] = run_scan()
@@ -316,6 +348,21 @@ def main():
                        timeout=120)
     copy_gbs = json.loads(r.stdout.strip().splitlines()[-1])["copy_GBs"] if r.returncode == 0 else None
 
+    system = ("AB diblock SCFT, RQM4 + Simpson + Anderson(20), 128^3, Ns=200, f=0.35, chiN=18, cubic a=4.2 Rg")
+    if head is batch:
+        workload = (f"{system}; {nsys} independent copies (replica noise streams) batched in one propagator "
+                    "program; FP32 propagators with FP64 Q/H/Anderson and the fused cell-stress adjoint")
+        step = f"one batched update: Model_AB::update of each of the {nsys} systems ({nsys} x 2*Ns contour steps)"
+        kernel = (f"RQM4 lockstep contour step of the batch (k_plane3 + k_xpass4 launches advancing the {2 * nsys} "
+                  "chains of the batch by one contour step, with the fused stress adjoint)")
+        launch_us = head["lockstep_group_us"]
+    else:
+        workload = system + ("; FP32 propagators with FP64 Q/H/Anderson and the fused cell-stress adjoint"
+                             if head is mixed else "; FP64 plain")
+        step = "one SCFT iteration (2*Ns contour steps)"
+        kernel = ("RQM4 lockstep contour step (k_plane3 + k_xpass3/k_xpass4 launches advancing both chains"
+                  + (", with the fused stress adjoint)" if head is mixed else ")"))
+        launch_us = head["lockstep_step_us"]
     line = {
         "metric": "contour-steps/s at 128^3 Ns=200: mixed+adjoint vs FP64 plain, |dH/H|<=1e-6",
         "value": head["contour_steps_per_s"],
@@ -327,37 +374,42 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if head is mixed else "f64",
+        "dtype": "f64" if head is f64 else "f32",
         "data": "synthetic (seeded double-gyroid template + Philox Gaussian noise)",
-        "config": {"workload": "AB diblock SCFT, RQM4 + Simpson + Anderson(20), 128^3, Ns=200, f=0.35, "
-                               "chiN=18, cubic a=4.2 Rg; " +
-                               ("FP32 propagators with FP64 Q/H/Anderson and the fused cell-stress adjoint"
-                                if head is mixed else "FP64 plain"),
-                   "grid": [128, 128, 128], "Ns": 200, "Ms": [71, 131],
-                   "step": "one SCFT iteration (2*Ns contour steps)", "parallelism": "single GPU",
+        "config": {"workload": workload, "grid": [128, 128, 128], "Ns": 200, "Ms": [71, 131],
+                   "systems": nsys if head is batch else 1, "step": step, "parallelism": "single GPU",
                    "launch_shape": head["launch_shape"]},
         "roofline": {"bound": "hbm", "achieved": head["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": head["achieved_GBs"] / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "RQM4 lockstep contour step (k_plane3 + k_xpass3/k_xpass4 launches advancing both chains"
-                               + (", with the fused stress adjoint)" if head is mixed else ")"),
-                     "bytes_per_launch": head["bytes_per_launch"], "avg_launch_us": head["lockstep_step_us"],
+                     "frac": head["achieved_GBs"] / HBM_PEAK_GBS, "traffic": traffic, "kernel": kernel,
+                     "bytes_per_launch": head["bytes_per_launch"], "avg_launch_us": launch_us,
                      "measured_copy_GBs": copy_gbs,
                      "frac_of_measured_copy": head["achieved_GBs"] / copy_gbs if copy_gbs else None},
         "cpu_baseline": cpu,
         "iters_per_s": head["iters_per_s"],
     }
+    if batch is not None:
+        line["stress_dHdl_replica0"] = batch["stress"][0]
     if mixed is not None:
-        line["stress_dHdl"] = mixed["stress"]
+        line["mixed_adjoint_single"] = {
+            "contour_steps_per_s": mixed["contour_steps_per_s"], "iters_per_s": mixed["iters_per_s"],
+            "lockstep_step_us": mixed["lockstep_step_us"], "achieved_GBs": mixed["achieved_GBs"],
+            "roofline_frac": mixed["achieved_GBs"] / HBM_PEAK_GBS, "stress_dHdl": mixed["stress"],
+            "launch_shape": mixed["launch_shape"]}
     if f64 is not None:
         line["fp64_plain"] = {"contour_steps_per_s": f64["contour_steps_per_s"], "iters_per_s": f64["iters_per_s"],
                               "lockstep_step_us": f64["lockstep_step_us"], "achieved_GBs": f64["achieved_GBs"],
-                              "roofline_frac": f64["achieved_GBs"] / HBM_PEAK_GBS, "H": f64["H"]}
+                              "roofline_frac": f64["achieved_GBs"] / HBM_PEAK_GBS, "H": f64["H"],
+                              "launch_shape": f64["launch_shape"]}
+        line["speedup_vs_fp64_plain"] = head["contour_steps_per_s"] / f64["contour_steps_per_s"]
+        if mixed is not None:
+            line["speedup_single_mixed_vs_fp64_plain"] = mixed["contour_steps_per_s"] / f64["contour_steps_per_s"]
     if mixed is not None and f64 is not None:
-        line["speedup_vs_fp64_plain"] = mixed["contour_steps_per_s"] / f64["contour_steps_per_s"]
         line["precision_error_same_fields"] = {
             "what": "FP32-bulk vs FP64 from the same initial fields: H at those fields (H_fixed), ln Q, "
                     "phi and the residual of the first update, and H after that update's Anderson step",
             "dH_over_H": abs(first32["H_fixed"] - first64["H_fixed"]) / abs(first64["H_fixed"]),
+            "dH_over_H_batch_replica0": (abs(batch["H_fixed_replica0"] - first64["H_fixed"]) / abs(first64["H_fixed"])
+                                         if batch is not None else None),
             "dlnQ": abs(np.log(first32["Q"]) - np.log(first64["Q"])),
             "dH_over_H_after_mixing": abs(first32["H"] - first64["H"]) / abs(first64["H"]),
             "max_dphiA": float(np.abs(first32["phiA"] - first64["phiA"]).max()),

@@ -788,7 +788,10 @@ k_xpass3(const XDesc<T> *__restrict__ descs, Roots<T> rt, int ncol, int hz) {
                                        : gload_nt(d.Sdot + sbase + (unsigned)((it * RX + r) * 64 + lane));
                     const int ky = c / hz, kz = c % hz;
                     const double wz = (kz == 0 || kz == hz - 1) ? 1.0 : 2.0;
-                    const double wk = wz * (double)f * ((double)y.x * (double)v[it][r].x + (double)y.y * (double)v[it][r].y);
+                    // FP32: the element product in FP32, the sums in FP64 (as k_xpass4)
+                    const double wk = PF ? wz * (double)(f * (y.x * v[it][r].x + y.y * v[it][r].y))
+                                         : wz * (double)f * ((double)y.x * (double)v[it][r].x +
+                                                             (double)y.y * (double)v[it][r].y);
                     acc[0] += wk * (double)gload(d.gx + x);
                     acc[1] += wk * (double)gload(d.gy + ky);
                     acc[2] += wk * (double)gload(d.gz + kz);
@@ -900,8 +903,13 @@ k_xpass4(const XDesc<T> *__restrict__ descs, Roots<T> rt, int ncol, int hz, int 
                     const cx<T> y0 = PF ? yd[0][PF ? it : 0][PF ? r : 0] : gload_nt(d.Sdot + so);
                     const cx<T> y1 = PF ? yd[PF ? 1 : 0][PF ? it : 0][PF ? r : 0]
                                         : gload_nt(d.Sdot + so + ITX * RX * 64);
-                    const double p0 = wz * (double)f0 * ((double)y0.x * (double)lo.x + (double)y0.y * (double)lo.y);
-                    const double p1 = wz * (double)f1 * ((double)y1.x * (double)hi.x + (double)y1.y * (double)hi.y);
+                    // FP32 forms each element's product in FP32 (one
+                    // conversion per product keeps the unrolled loop's live
+                    // doubles, and the registers, down); the sums stay FP64
+                    const double p0 = PF ? wz * (double)(f0 * (y0.x * lo.x + y0.y * lo.y))
+                                         : wz * (double)f0 * ((double)y0.x * (double)lo.x + (double)y0.y * (double)lo.y);
+                    const double p1 = PF ? wz * (double)(f1 * (y1.x * hi.x + y1.y * hi.y))
+                                         : wz * (double)f1 * ((double)y1.x * (double)hi.x + (double)y1.y * (double)hi.y);
                     const double gx = gload(d.gx + x);
                     acc[0] += (p0 + p1) * gx;
                     acc[1] += p0 * (double)gload(d.gy + kp) + p1 * (double)gload(d.gy + kp + lyh);

@@ -933,6 +933,7 @@ template <typename T> struct Batch : po_batch {
     std::vector<Model<T> *> ms;
     Program<T> prog;
     bool adjoint = false;
+    bool dual = true;  // forward and complementary chains on two streams
     hipStream_t st2 = nullptr;
     hipEvent_t dev[4] = {};
 
@@ -977,6 +978,9 @@ template <typename T> struct Batch : po_batch {
         prog.hold = m0.prog.hold;
         steps = m0.steps * n;
         lockstep = m0.steps / 2;
+        // PO_DUAL=0: every lane of a launch on the context's stream
+        const char *env = std::getenv("PO_DUAL");
+        dual = !(env && env[0] == '0');
         bool ok = hipStreamCreateWithFlags(&st2, hipStreamNonBlocking) == hipSuccess;
         for (auto &e : dev) ok = ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
         if (!ok) return fail(PO_ERR_DEVICE, "batch stream / event creation failed");
@@ -993,12 +997,17 @@ template <typename T> struct Batch : po_batch {
         const long rows = adjoint ? (long)prog.ndot * xpass_blocks(m0.sh, prog.x_variant, prog.split) : 0;
         for (int it = 0; it < iters; ++it) {
             PO_HIP(mark());
-            PO_HIP(hipEventRecord(dev[0], st));
-            PO_HIP(hipStreamWaitEvent(st2, dev[0], 0));
-            int rc = prog.run_dual(m0.sh, m0.rt, st, st2, dev[1], dev[2]);
-            if (rc) return rc;
-            PO_HIP(hipEventRecord(dev[3], st2));
-            PO_HIP(hipStreamWaitEvent(st, dev[3], 0));
+            int rc;
+            if (dual) {
+                PO_HIP(hipEventRecord(dev[0], st));
+                PO_HIP(hipStreamWaitEvent(st2, dev[0], 0));
+                rc = prog.run_dual(m0.sh, m0.rt, st, st2, dev[1], dev[2]);
+                if (rc) return rc;
+                PO_HIP(hipEventRecord(dev[3], st2));
+                PO_HIP(hipStreamWaitEvent(st, dev[3], 0));
+            } else if ((rc = prog.run(m0.sh, m0.rt, st))) {
+                return rc;
+            }
             PO_HIP(mark());
             for (auto *m : ms)
                 if ((rc = m->post_update(rows))) return rc;
@@ -1013,7 +1022,7 @@ template <typename T> struct Batch : po_batch {
         o[1] = prog.plane_variant;
         o[2] = prog.x_variant;
         o[3] = prog.hold;
-        o[4] = 1;
+        o[4] = dual ? 1 : 0;
     }
 
     ~Batch() override {

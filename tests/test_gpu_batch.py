@@ -119,3 +119,26 @@ def test_batch_rejects_mismatched_models(ctx64, ctx32):
         assert e.value.code == capi.PO_ERR_INVALID
     with pytest.raises(capi.POError):
         capi.Batch([m0, m0])
+
+
+def test_batch_one_stream_matches_two(ctx32, monkeypatch):
+    """PO_DUAL=0 runs every lane of each batched launch on one stream: the
+    same kernels on the same descriptors, so the same bits as the default
+    two-stream schedule."""
+    for k in PINS:
+        monkeypatch.setenv(k, "0")
+    monkeypatch.setenv("PO_SPLIT", "1")
+    out = []
+    for dual in ("1", "0"):
+        monkeypatch.setenv("PO_DUAL", dual)
+        models = []
+        for s in SYSTEMS:
+            wA, wB, kw = make(ctx32, s, "RQM4")
+            models.append(capi.ModelAB(ctx32, wA, wB, **kw))
+        b = capi.Batch(models, adjoint=True)
+        assert b.launch_shape()["PO_DUAL"] == int(dual)
+        b.update(2)
+        out.append([(m.observe(), m.last_stress().tolist(), m.get("wA")) for m in models])
+        b.close()
+    for (o1, s1, w1), (o0, s0, w0) in zip(*out):
+        assert o1 == o0 and s1 == s0 and np.array_equal(w1, w0)

@@ -259,8 +259,10 @@ int po_model_launch_shape(po_model *m, int *out5);
  * model keeps its own fields, Anderson history and observables, read with
  * the po_model_* calls.  With adjoint != 0 every batched update also fuses
  * the cell stress (po_model_last_stress of each model, as
- * po_model_set_adjoint).  The models stay usable alone; a batch must be
- * destroyed before its models. */
+ * po_model_set_adjoint).  The models stay usable alone.  While a batch
+ * exists its models refuse po_model_destroy, and a po_model_set_adjoint that
+ * would reallocate the adjoint arena, with PO_ERR_STATE: destroy the batch
+ * first. */
 typedef struct po_batch po_batch;
 int po_batch_create(po_model **models, int n, int adjoint, po_batch **out);
 int po_batch_destroy(po_batch *b);

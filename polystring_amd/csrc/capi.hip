@@ -146,6 +146,7 @@ struct po_model {
     po_ctx *ctx = nullptr;
     po_model_desc desc{};
     long steps = 0;
+    int in_batch = 0;  // batches whose launch program points into this model's buffers
     // profiling: three events per update() -- before the propagator chains,
     // after them, and after the density / Anderson kernels
     bool profiling = false;
@@ -488,6 +489,9 @@ template <typename T> struct Model : po_model {
         a.elems = elems;
         a.nblk = xpass_blocks(sh, xv, split);
         const size_t ns = (size_t)(N / 2) * 2 * nk * elems, np = (size_t)(N / 2) * 2 * nk * a.nblk * 3;
+        if ((fusedSpec.n != ns || fusedPart.n != np) && in_batch > 0)
+            return fail(PO_ERR_STATE, "the model's adjoint arena is in use by a batch with another tile layout; "
+                                      "destroy the batch first");
         // the model's own fused program points into these buffers: drop it,
         // update() rebuilds it on its next fused run
         if (fusedSpec.n != ns || fusedPart.n != np) prog_adj.reset();
@@ -933,7 +937,8 @@ template <typename T> struct Batch : po_batch {
     std::vector<Model<T> *> ms;
     Program<T> prog;
     bool adjoint = false;
-    bool dual = true;  // forward and complementary chains on two streams
+    bool dual = true;         // forward and complementary chains on two streams
+    bool registered = false;  // the models count this batch (init succeeded)
     hipStream_t st2 = nullptr;
     hipEvent_t dev[4] = {};
 
@@ -984,6 +989,8 @@ template <typename T> struct Batch : po_batch {
         bool ok = hipStreamCreateWithFlags(&st2, hipStreamNonBlocking) == hipSuccess;
         for (auto &e : dev) ok = ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
         if (!ok) return fail(PO_ERR_DEVICE, "batch stream / event creation failed");
+        for (auto *m : ms) ++m->in_batch;
+        registered = true;
         return PO_OK;
     }
 
@@ -1026,6 +1033,8 @@ template <typename T> struct Batch : po_batch {
     }
 
     ~Batch() override {
+        if (registered)
+            for (auto *m : ms) --m->in_batch;
         if (st2) {
             (void)hipStreamSynchronize(st2);
             (void)hipStreamDestroy(st2);
@@ -1202,6 +1211,7 @@ int po_model_create(po_ctx *ctx, const po_model_desc *desc, const double *wA0, c
 }
 
 int po_model_destroy(po_model *m) {
+    if (m && m->in_batch > 0) return fail(PO_ERR_STATE, "the model belongs to a batch: destroy the batch first");
     // hipFree in the member destructors waits for outstanding work; the
     // context is not touched, so a model may outlive its context's handle.
     delete m;

@@ -142,3 +142,22 @@ def test_batch_one_stream_matches_two(ctx32, monkeypatch):
         b.close()
     for (o1, s1, w1), (o0, s0, w0) in zip(*out):
         assert o1 == o0 and s1 == s0 and np.array_equal(w1, w0)
+
+
+def test_batch_members_refuse_destroy(ctx64):
+    """A model whose buffers a batch's program points into cannot be
+    destroyed before the batch; afterwards it can."""
+    models = []
+    for s in SYSTEMS[:2]:
+        wA, wB, kw = make(ctx64, s, "RQM4")
+        models.append(capi.ModelAB(ctx64, wA, wB, **kw))
+    b = capi.Batch(models, adjoint=True)
+    lib = capi.load()
+    assert lib.po_model_destroy(models[0].h) == capi.PO_ERR_STATE
+    b.update(1)
+    models[0].set_adjoint(True)  # same tile layout as the batch: no reallocation
+    models[0].update(1)
+    b.update(1)
+    b.close()
+    assert lib.po_model_destroy(models[0].h) == capi.PO_OK
+    models[0].h = None

@@ -1,0 +1,52 @@
+"""Long-run FP64 parity: 1000 Anderson iterations of the 32^3 AB melt (f = 0.3,
+chiN = 20, cubic a = 4.0, Ns = 100, Ms = [31, 71], history 20, lam 0.1, seeded
+noisy homogeneous start) on the GPU against the FP64 oracle's end state after
+the same 1000 iterations (tests/golden/cfg1_relax1000.npz, written by
+tests/golden/make_cfg1_relax.py; the oracle needs about 13 minutes for it).
+
+At chiN = 20 this melt relaxes to the disordered state: phi_A = f everywhere
+and H = chiN f (1 - f) = 4.2 exactly.  Anderson mixing approaches it slowly
+(residual about 5e-7 after 1000 iterations) and amplifies rounding along the
+way: two oracle runs whose threaded sums round differently end 0.7% apart in
+the residual, and the GPU run ends with max|dwA| = 7e-4 against the oracle
+at residuals 9.6e-7 (GPU) and 4.6e-7 (oracle).  What the fixed point fixes is
+compared tightly; the fields are compared with the fixed point itself:
+  * H against the oracle's: measured 1.3e-11 relative (bar 1e-10);
+  * ln Q against the oracle's: measured 1.8e-10 absolute (bar 1e-9);
+  * H against the closed form 4.2: measured 6e-11 (bar 1e-9);
+  * max|phi_A - f|: measured 2.6e-6 (oracle), about 1e-5 (GPU) (bar 1e-4);
+    <phi_A> = f to 1e-12 (the quadrature invariant);
+  * both residuals below 2e-6.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from polystring_amd import capi, workloads
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def test_cfg1_1000_iterations_match_oracle(ctx64):
+    g = np.load(os.path.join(GOLD, "cfg1_relax1000.npz"))
+    wl = workloads.cfg1()
+    f = wl.f
+    gm = capi.ModelAB(ctx64, wl.wA, wl.wB, **wl.model_kwargs())
+    it, H, res = gm.relax(1000, 1000, 1000, 0.0)
+    o = gm.observe()
+    phiA = gm.get("phiA")
+    Ho, reso = float(g["H"]), float(g["residual"])
+    dH = abs(H - Ho) / abs(Ho)
+    dlnQ = abs(np.log(o["Q"]) - float(g["lnQ"]))
+    print(f"\n[cfg1 x1000] it {it} H {H!r} oracle {Ho!r} |dH/H| {dH:.2e} dlnQ {dlnQ:.2e} "
+          f"max|phiA - f| gpu {np.abs(phiA - f).max():.2e} oracle {np.abs(g['phiA'] - f).max():.2e} "
+          f"residual gpu {res:.3e} oracle {reso:.3e}", flush=True)
+    assert it == 1000 and int(g["iters"]) == 1000
+    assert dH <= 1e-10
+    assert dlnQ <= 1e-9
+    assert abs(H - 4.2) <= 1e-9 and abs(Ho - 4.2) <= 1e-9
+    assert np.abs(phiA - f).max() <= 1e-4 and np.abs(g["phiA"] - f).max() <= 1e-4
+    assert abs(phiA.mean() - f) <= 1e-12
+    assert res < 2e-6 and reso < 2e-6

@@ -24,7 +24,7 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
          "-Wall", "-Wno-unused-command-line-argument"]
 SOURCES = ["propagator.hip", "kernels.hip", "capi.hip", "string.hip"]
 HOST = os.path.join(HERE, "host")
-HOST_SOURCES = ["mat5.cc", "Config.cc", "Propagator.cc", "Model_AB.cc", "scft.cc"]
+HOST_SOURCES = ["mat5.cc", "Config.cc", "Propagator.cc", "Model_AB.cc", "ModelBatch.cc", "scft.cc"]
 HOST_LIB = os.path.join(HERE, "libpolyorder_host.so")
 DRIVER = os.path.join(HERE, "polyorder_amd")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")

@@ -1,0 +1,19 @@
+// ModelBatch.cc -- see ModelBatch.h.
+#include "ModelBatch.h"
+
+#include "Propagator.h"
+
+ModelBatch::ModelBatch(const std::vector<Model_AB *> &models, bool adjoint) : _ms(models) {
+    std::vector<po_model *> hs;
+    for (auto *m : _ms) hs.push_back(m->handle());
+    po_host_check(po_batch_create(hs.data(), (int)hs.size(), adjoint ? 1 : 0, &_b), "po_batch_create");
+}
+
+ModelBatch::~ModelBatch() {
+    if (_b) po_batch_destroy(_b);
+}
+
+void ModelBatch::update(int n) {
+    po_host_check(po_batch_update(_b, n), "po_batch_update");
+    for (auto *m : _ms) m->invalidate();
+}

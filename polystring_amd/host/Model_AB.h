@@ -34,6 +34,9 @@ public:
     double Q() const;
     std::vector<double> field(int which) const;  // PO_FIELD_WA / WB / PHIA / PHIB
     po_model *handle() const { return _m; }
+    // the device state changed outside update() (a batched update): the
+    // cached observables are re-read on the next query
+    void invalidate() { _obs_valid = false; }
 
 private:
     void observe() const;

@@ -139,3 +139,49 @@ def test_driver_runs_etdrk4(tmp_path):
     done = dict(l.split(":\t", 1) for l in (tmp_path / "done.txt").read_text().splitlines() if ":\t" in l)
     Hpub = json.load(open(os.path.join(GOLD, "known_answers.json")))["1D"]["ETDRK4+Anderson"]["H"]
     assert abs(float(done["H"]) - Hpub) < 1e-6
+
+
+BATCH_CC = r"""
+#include <cstdio>
+#include "Model_AB.h"
+#include "ModelBatch.h"
+int main() {
+    Model_AB a("param.ini"), b("param.ini"), alone("param.ini");
+    {
+        ModelBatch batch({&a, &b});
+        for (int i = 0; i < 5; ++i) {
+            batch.update(1);
+            alone.update();
+            std::printf("%.17g %.17g %.17g\n", a.H(), b.H(), alone.H());
+        }
+    }
+    a.update();  // the models stay usable once the batch is gone
+    alone.update();
+    std::printf("%.17g %.17g\n", a.H(), alone.H());
+    return 0;
+}
+"""
+
+
+@pytest.mark.gpu
+def test_cpp_model_batch_matches_models_alone(tmp_path):
+    """ModelBatch (C++ mirror over po_batch_*): two Model_AB built from one
+    param.ini and updated as a batch give, at every update, the same H as a
+    third updated alone (launch shapes pinned)."""
+    host = os.path.join(PKG, "host")
+    src, exe = tmp_path / "batch.cc", tmp_path / "batch"
+    src.write_text(BATCH_CC)
+    subprocess.check_call(["g++", "-O1", "-std=c++14", f"-I{host}", f"-I{os.path.join(ROOT, 'include')}", str(src),
+                           f"-L{PKG}", "-lpolyorder_host", "-lpolystring_amd", f"-Wl,-rpath,{PKG}", "-o", str(exe)])
+    f = np.load(os.path.join(GOLD, "lam1d_field.npz"))
+    sio.savemat(str(tmp_path / "field_in_64.mat"), {"wA": f["wA"], "wB": f["wB"]})
+    (tmp_path / "param.ini").write_text(INI)
+    env = dict(os.environ, PO_SPLIT="0", PO_PLANE_VARIANT="0", PO_XPASS_VARIANT="0", PO_PLANE_HOLD="0")
+    print("\n[host] running the ModelBatch program", flush=True)
+    out = subprocess.run([str(exe)], cwd=tmp_path, capture_output=True, text=True, timeout=120, env=env)
+    assert out.returncode == 0, out.stdout + out.stderr
+    lines = [l.split() for l in out.stdout.strip().splitlines()]
+    assert len(lines) == 6
+    for l in lines[:5]:
+        assert l[0] == l[1] == l[2], l
+    assert lines[5][0] == lines[5][1]

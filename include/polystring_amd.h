@@ -232,6 +232,13 @@ int po_string_iterate(po_string *s, int max_iter, int min_iter, int record_inter
 int po_string_get(po_string *s, int which, double *host_out);
 /* contour steps of one EM update of all images (2 m Ns) */
 long po_string_contour_steps(const po_string *s);
+/* Per-image cell sizes, the string driver's is_changeSize
+ * (scft/polystring_AB.cc:85-91; Model_AB::resetInString, src/Model_AB.cc:52-83):
+ * image i takes size[i] (m values) as its cell length a for 1-D and 2-D
+ * strings, b for 3-D ones; the crystal system ties the other lengths.  The
+ * driver's sizes are size_S + i/(m-1) (size_E - size_S); without
+ * is_changeSize it passes a to every image, which for a 3-D string sets b = a. */
+int po_string_set_sizes(po_string *s, const double *size);
 
 /* Measurement hooks; no counterpart in the reference (which records CPU
  * clock() per iteration, src/scft.cc:398-412).  With profiling on, every

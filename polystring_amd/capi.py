@@ -47,7 +47,7 @@ EXPORTS = [
     "po_string_contour_steps", "po_model_set_graph", "po_models_relax",
     "po_model_set_profiling", "po_model_profile", "po_model_launch_shape",
     "po_batch_create", "po_batch_destroy", "po_batch_update", "po_batch_contour_steps",
-    "po_batch_set_profiling", "po_batch_profile", "po_batch_launch_shape",
+    "po_batch_set_profiling", "po_batch_profile", "po_batch_launch_shape", "po_string_set_sizes",
 ]
 
 
@@ -120,6 +120,7 @@ def load():
         "po_batch_update": ([_vp, i], i), "po_batch_contour_steps": ([_vp], ctypes.c_long),
         "po_batch_set_profiling": ([_vp, i], i), "po_batch_profile": ([_vp, _dp], i),
         "po_batch_launch_shape": ([_vp, ctypes.POINTER(ctypes.c_int)], i),
+        "po_string_set_sizes": ([_vp, _dp], i),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -434,3 +435,10 @@ class String:
 
     def contour_steps(self):
         return load().po_string_contour_steps(self.h)
+
+    def set_sizes(self, sizes):
+        """Per-image cell length (a for 1-D/2-D strings, b for 3-D): is_changeSize."""
+        sizes = np.ascontiguousarray(sizes, dtype=np.float64)
+        if sizes.shape != (self.m,):
+            raise ValueError(f"need {self.m} sizes")
+        _check(load().po_string_set_sizes(self.h, _p(sizes)))

@@ -3,6 +3,8 @@
 // program, with batched simple-mixing (EM) updates, arc-length
 // parametrisation and a cubic-spline redistribution of the images
 // (scft/polystring_AB.cc:111-329; Model_AB EM branch src/Model_AB.cc:136-261).
+#include <memory>
+
 #include "runtime.hpp"
 
 namespace po {
@@ -236,6 +238,7 @@ struct po_string {
     virtual int iterate(int max_iter, int min_iter, int record_interval, double thresh, double *H, double *S) = 0;
     virtual int get(int which, double *out) = 0;
     virtual long contour_steps() const = 0;
+    virtual int set_sizes(const double *size) = 0;
 };
 
 namespace po {
@@ -251,6 +254,8 @@ template <typename T> struct StringAB : po_string {
     DBuf<T> eA2, eA4, eB2, eB4, qA, qB, qAc, qBc;
     DBuf<cx<T>> spec;
     KTable<T> kA, kAh, kB, kBh;
+    // per-image cells (set_sizes): image i's tables kimg[4i .. 4i+3] = A, A half, B, B half
+    std::vector<std::unique_ptr<KTable<T>>> kimg;
     Program<T> prog;
 
     int init(po_ctx *c, const po_string_desc &d, const double *hA, const double *hB) {
@@ -321,26 +326,75 @@ template <typename T> struct StringAB : po_string {
             if ((rc = build_ktable<T>(kAh, sh, cell, false, 0.5 * dsA, st))) return rc;
             if ((rc = build_ktable<T>(kBh, sh, cell, false, 0.5 * dsB, st))) return rc;
         }
-        // lanes 2i: qA_i -> qB_i, 2i+1: qBc_i -> qAc_i (src/Model_AB.cc:139-145)
+        for (int i = 0; i < m; ++i) {
+            PO_HIP(launch_fill<T>(qA.p + (size_t)i * sA * M, (T)1, M, st));   // heads qA[0] = qBc[0] = 1
+            PO_HIP(launch_fill<T>(qBc.p + (size_t)i * sB * M, (T)1, M, st));  // (src/Model_AB.cc:779-782)
+        }
+        prog.slice_elems = M;
+        if ((rc = build_program())) return rc;
+        return prog.tune(sh, rt, st);
+    }
+
+    // The launch program: lanes 2i: qA_i -> qB_i, 2i+1: qBc_i -> qAc_i
+    // (src/Model_AB.cc:139-145), with the string's k tables or, after
+    // set_sizes, each image's own.
+    int build_program() {
+        const long M = sh.M;
         std::vector<Lane<T>> lanes(2 * m);
         for (int i = 0; i < m; ++i) {
             const size_t o = (size_t)i * M;
             T *a = qA.p + (size_t)i * sA * M, *b = qB.p + (size_t)i * sB * M;
             T *ac = qAc.p + (size_t)i * sA * M, *bc = qBc.p + (size_t)i * sB * M;
-            lanes[2 * i].segs = {Segment<T>{a, sA, eA2.p + o, eA4.p + o, &kA, &kAh},
-                                 Segment<T>{b, sB, eB2.p + o, eB4.p + o, &kB, &kBh}};
-            lanes[2 * i + 1].segs = {Segment<T>{bc, sB, eB2.p + o, eB4.p + o, &kB, &kBh},
-                                     Segment<T>{ac, sA, eA2.p + o, eA4.p + o, &kA, &kAh}};
+            const KTable<T> *ka = &kA, *kah = &kAh, *kb = &kB, *kbh = &kBh;
+            if (!kimg.empty()) {
+                ka = kimg[4 * i].get();
+                kah = kimg[4 * i + 1].get();
+                kb = kimg[4 * i + 2].get();
+                kbh = kimg[4 * i + 3].get();
+            }
+            lanes[2 * i].segs = {Segment<T>{a, sA, eA2.p + o, eA4.p + o, ka, kah},
+                                 Segment<T>{b, sB, eB2.p + o, eB4.p + o, kb, kbh}};
+            lanes[2 * i + 1].segs = {Segment<T>{bc, sB, eB2.p + o, eB4.p + o, kb, kbh},
+                                     Segment<T>{ac, sA, eA2.p + o, eA4.p + o, ka, kah}};
             for (int l = 0; l < 2; ++l) {
                 lanes[2 * i + l].Sf = spec.p + (size_t)(4 * i + 2 * l) * sh.Mh;
                 lanes[2 * i + l].Sh = spec.p + (size_t)(4 * i + 2 * l + 1) * sh.Mh;
             }
-            PO_HIP(launch_fill<T>(a, (T)1, M, st));   // heads qA[0] = qBc[0] = 1
-            PO_HIP(launch_fill<T>(bc, (T)1, M, st));  // (src/Model_AB.cc:779-782)
         }
-        prog.slice_elems = M;
-        if ((rc = prog.build(d.algo, lanes, st))) return rc;
-        return prog.tune(sh, rt, st);
+        prog.reset();
+        return prog.build(desc.algo, lanes, ctx->stream);
+    }
+
+    // is_changeSize (scft/polystring_AB.cc:85-91 with Model_AB::resetInString,
+    // src/Model_AB.cc:52-83): image i takes the cell length size[i] for a
+    // (1-D and 2-D strings) or b (3-D); the crystal system ties the other
+    // lengths as UnitCell does.  The launch shape stays the tuned one.
+    int set_sizes(const double *size) override {
+        hipStream_t st = ctx->stream;
+        PO_HIP(hipStreamSynchronize(st));
+        std::vector<std::unique_ptr<KTable<T>>> k(4 * (size_t)m);
+        const bool general = (desc.algo == PO_OS2) && !cell.ortho;
+        int rc;
+        for (int i = 0; i < m; ++i) {
+            if (!(size[i] > 0) || !std::isfinite(size[i])) return fail(PO_ERR_INVALID, "image sizes must be positive");
+            po_grid_desc g = desc.grid;
+            (g.dim == 3 ? g.b : g.a) = size[i];
+            Cell ci{};
+            if ((rc = make_cell(g, ci))) return rc;
+            if (ci.ortho != cell.ortho) return fail(PO_ERR_INVALID, "image cells must keep the string's cell type");
+            for (int t = 0; t < 4; ++t) k[4 * i + t].reset(new KTable<T>());
+            if ((rc = build_ktable<T>(*k[4 * i], sh, ci, general, dsA, st))) return rc;
+            if ((rc = build_ktable<T>(*k[4 * i + 2], sh, ci, general, dsB, st))) return rc;
+            if (desc.algo == PO_RQM4) {
+                if ((rc = build_ktable<T>(*k[4 * i + 1], sh, ci, false, 0.5 * dsA, st))) return rc;
+                if ((rc = build_ktable<T>(*k[4 * i + 3], sh, ci, false, 0.5 * dsB, st))) return rc;
+            }
+        }
+        PO_HIP(hipStreamSynchronize(st));
+        kimg.swap(k);  // the previous per-image tables (now in k) are freed after the rebuild
+        rc = build_program();
+        PO_HIP(hipStreamSynchronize(st));
+        return rc;
     }
 
     int factors() {
@@ -529,5 +583,11 @@ int po_string_get(po_string *s, int which, double *host_out) {
 }
 
 long po_string_contour_steps(const po_string *s) { return s ? s->contour_steps() : -1; }
+
+int po_string_set_sizes(po_string *s, const double *size) {
+    if (!s || !size) return fail(PO_ERR_INVALID, "null argument");
+    PO_HIP(hipSetDevice(s->ctx->device));
+    return s->set_sizes(size);
+}
 
 }  // extern "C"

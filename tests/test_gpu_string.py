@@ -106,3 +106,46 @@ def test_ragged_scan_matches_independent_runs(ctx64, monkeypatch):
         it1, H1, r1 = alone.relax(6, 2, 2, 1e-12)
         print(f"  {w.name} {w.shape} Ns={w.Ns}: scan H {H[k]!r} alone {H1!r}", flush=True)
         assert it1 == 6 and H[k] == H1 and r[k] == r1
+
+
+SIZE_CASES = [
+    # shape, cell(size) -> cell tuple, dim, Ms, compressible, mirror, lam
+    ((32, 32, 1), lambda a: ("RECTANGULAR", a, 4.4), 2, (11, 11), True, True, (0.015, 0.015, 100.0)),
+    ((16, 16, 16), lambda b: ("ORTHORHOMBIC", 4.0, b, 4.2), 3, (9, 13), False, False, (0.1, 0.1, 1.0)),
+    ((64, 1, 1), lambda a: ("LAMELLAR", a), 1, (11, 11), False, True, (0.05, 0.05, 5.0)),
+]
+
+
+@pytest.mark.parametrize("shape,cellf,dim,Ms,compressible,mirror,lam", SIZE_CASES)
+def test_string_per_image_sizes_match_oracle(ctx64, shape, cellf, dim, Ms, compressible, mirror, lam):
+    """is_changeSize (scft/polystring_AB.cc:85-91, src/Model_AB.cc:52-83):
+    image s takes size_S + s/(m-1) (size_E - size_S) as its cell length a
+    (1-D, 2-D) or b (3-D).  The oracle runs each image with that cell."""
+    m, chiN = 5, 15.0
+    size_S, size_E = 3.7, 4.5
+    sizes = size_S + np.arange(m) / (m - 1) * (size_E - size_S)
+    print(f"\n[string sizes] {shape} sizes {sizes.tolist()}", flush=True)
+    wa, wb = initial_string(shape, m, chiN, 0.5)
+    relax = (3, 1, 1, 1e-10)
+    models = [O.EMModel(wa[s], wb[s], dim=dim, cell=cellf(sizes[s]), Ms=Ms, chiN=chiN, lam=lam, mirror=mirror,
+                        compressible=compressible) for s in range(m)]
+    ends = (wa[0].copy(), wb[0].copy(), wa[-1].copy(), wb[-1].copy())
+    g = capi.String(ctx64, wa, wb, cell=cellf(4.0), Ms=Ms, chiN=chiN, lam=lam, compressible=compressible,
+                    mirror=mirror, dim=dim)
+    g.set_sizes(sizes)
+    owa, owb = wa, wb
+    for it in range(2):
+        H, S = g.iterate(*relax)
+        Ho, So, owa, owb, opa, opb = O.string_iterate(models, owa, owb, ends, relax)
+        print(f"  it {it}: H {H} oracle {Ho}", flush=True)
+        assert relerr(H, Ho) <= 1e-10
+        assert np.abs(S - So).max() <= 1e-10
+        assert relerr(g.get("wA"), owa) <= 1e-10
+        assert relerr(g.get("phiA"), opa) <= 1e-10
+    # the sizes matter: the string's images now differ from a string at the base cell
+    base = capi.String(ctx64, wa, wb, cell=cellf(4.0), Ms=Ms, chiN=chiN, lam=lam, compressible=compressible,
+                       mirror=mirror, dim=dim)
+    Hb, _ = base.iterate(*relax)
+    assert relerr(Hb, Ho) > 1e-6
+    with pytest.raises(capi.POError):
+        g.set_sizes(np.full(m, -1.0))

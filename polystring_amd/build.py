@@ -27,6 +27,7 @@ HOST = os.path.join(HERE, "host")
 HOST_SOURCES = ["mat5.cc", "Config.cc", "Propagator.cc", "Model_AB.cc", "ModelBatch.cc", "scft.cc"]
 HOST_LIB = os.path.join(HERE, "libpolyorder_host.so")
 DRIVER = os.path.join(HERE, "polyorder_amd")
+STRING_DRIVER = os.path.join(HERE, "polystring_AB_amd")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
 
@@ -52,7 +53,7 @@ def build(verbose=True):
     os.replace(tmp, LIB)
     build_host()
     if verbose:
-        print(f"built {LIB}, {HOST_LIB}, {DRIVER}")
+        print(f"built {LIB}, {HOST_LIB}, {DRIVER}, {STRING_DRIVER}")
     return LIB
 
 
@@ -74,6 +75,11 @@ def build_host():
     if r.returncode != 0:
         raise RuntimeError("driver build failed:\n" + r.stdout + r.stderr)
     os.replace(DRIVER + ".tmp", DRIVER)
+    r = run([cxx] + common + ["-o", STRING_DRIVER + ".tmp", os.path.join(HOST, "polystring_AB_amd.cc"),
+                              f"-L{HERE}", "-lpolyorder_host", "-lpolystring_amd", "-Wl,-rpath,$ORIGIN"])
+    if r.returncode != 0:
+        raise RuntimeError("string driver build failed:\n" + r.stdout + r.stderr)
+    os.replace(STRING_DRIVER + ".tmp", STRING_DRIVER)
 
 
 if __name__ == "__main__":

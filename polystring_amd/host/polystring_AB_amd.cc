@@ -1,0 +1,227 @@
+// polystring_AB_amd.cc -- command-line string-method driver, the flow of the
+// reference's scft/polystring_AB.cc with every image on the GPU
+// (po_string_*):
+//   paramString.ini -> initial string (two end states, or a saved string)
+//   -> string iterations (EM relax of every image, arc length, spline
+//   redistribution, pinned ends) until max dH <= thresh_string_H
+//   -> result.mat every save_string_interval iterations (as the reference,
+//   nothing is saved at exit).
+#include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <iostream>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "Config.h"
+#include "Propagator.h"
+#include "UnitCell.h"
+#include "mat5.h"
+#include "polystring_amd.h"
+
+namespace {
+
+std::string lower(std::string s) {
+    for (auto &c : s) c = (char)std::tolower((unsigned char)c);
+    return s;
+}
+
+// One 3-D variable of a MAT file in C order (input_data3, scft/polystring_AB.cc:153-165).
+std::vector<double> read3(const std::string &file, const std::string &name, int Lx, int Ly, int Lz) {
+    std::map<std::string, mat5::Var> vars;
+    std::string err;
+    if (!mat5::read(file, vars, err) || !vars.count(name)) {
+        std::cerr << "polystring: cannot read " << name << " from " << file << ": " << err << std::endl;
+        std::exit(1);
+    }
+    const mat5::Var &v = vars[name];
+    if (v.data.size() != (size_t)Lx * Ly * Lz) {
+        std::cerr << "polystring: " << name << " in " << file << " does not match Lx*Ly*Lz" << std::endl;
+        std::exit(1);
+    }
+    return mat5::fortran_to_c(v.data.data(), Lx, Ly, Lz);
+}
+
+// One 4-D (m, Lx, Ly, Lz) variable in image-major C order (input_data4, :167-179).
+std::vector<double> read4(const std::string &file, const std::string &name, int m, int Lx, int Ly, int Lz) {
+    std::map<std::string, mat5::Var> vars;
+    std::string err;
+    const size_t M = (size_t)Lx * Ly * Lz;
+    if (!mat5::read(file, vars, err) || !vars.count(name) || vars[name].data.size() != (size_t)m * M) {
+        std::cerr << "polystring: cannot read " << name << " (m x Lx x Ly x Lz) from " << file << ": " << err
+                  << std::endl;
+        std::exit(1);
+    }
+    const std::vector<double> &f = vars[name].data;  // (s, i, j, k) at s + m (i + Lx (j + Ly k))
+    std::vector<double> c((size_t)m * M);
+    for (int s = 0; s < m; ++s)
+        for (int i = 0; i < Lx; ++i)
+            for (int j = 0; j < Ly; ++j)
+                for (int k = 0; k < Lz; ++k)
+                    c[(size_t)s * M + ((size_t)i * Ly + j) * Lz + k] = f[s + (size_t)m * (i + (size_t)Lx * (j + (size_t)Ly * k))];
+    return c;
+}
+
+std::vector<double> to_fortran4(const std::vector<double> &c, int m, int Lx, int Ly, int Lz) {
+    const size_t M = (size_t)Lx * Ly * Lz;
+    std::vector<double> f(c.size());
+    for (int s = 0; s < m; ++s)
+        for (int i = 0; i < Lx; ++i)
+            for (int j = 0; j < Ly; ++j)
+                for (int k = 0; k < Lz; ++k)
+                    f[s + (size_t)m * (i + (size_t)Lx * (j + (size_t)Ly * k))] = c[(size_t)s * M + ((size_t)i * Ly + j) * Lz + k];
+    return f;
+}
+
+void check(int rc, const char *what) {
+    if (rc != PO_OK) {
+        std::cerr << "polystring: " << what << " failed: " << po_last_error() << std::endl;
+        std::exit(1);
+    }
+}
+
+}  // namespace
+
+int main(int argc, char *argv[]) {
+    const std::string config_file = argc > 1 ? argv[1] : "paramString.ini";  // scft/polystring_AB.cc:52,97-99
+    Config cfg(config_file.c_str());
+    // [string] keys (scft/polystring_AB.cc:71-80)
+    const int m = cfg.get_int("string", "num_of_beads");
+    const int maxT = cfg.get_int("string", "string_iteration");
+    const int save_string_interval = std::max(1, cfg.get_int("string", "save_string_interval", 1));
+    const bool is_readString = cfg.get_bool("string", "is_readString");
+    const bool is_changeSize = cfg.get_bool("string", "is_changeSize");
+    const double thresh_string_H = cfg.get_double("string", "thresh_string_H");
+    const double size_S = cfg.get_double("string", "size_S"), size_E = cfg.get_double("string", "size_E");
+    if (m < 3 || maxT < 1) {
+        std::cerr << "polystring: num_of_beads must be >= 3 and string_iteration >= 1" << std::endl;
+        return 1;
+    }
+    // per-image cell length (:85-91): linear in the image index with
+    // is_changeSize, else a for every image (Model_AB::resetInString then
+    // sets a in 1-D / 2-D and b in 3-D, src/Model_AB.cc:52-83)
+    std::vector<double> stringSize(m);
+    for (int s = 0; s < m; ++s)
+        stringSize[s] = is_changeSize ? size_S + 1.0 * s / (m - 1) * (size_E - size_S) : cfg.a();
+    std::cout << "X or Y physical size along the string is";
+    for (double v : stringSize) std::cout << " " << v;
+    std::cout << std::endl;
+
+    const int Lx = (int)cfg.Lx(), Ly = (int)cfg.Ly(), Lz = (int)cfg.Lz();
+    const size_t M = (size_t)Lx * Ly * Lz;
+    // initial string (initialize_string, :181-222)
+    std::vector<double> wa, wb, phia, phib;
+    if (is_readString) {
+        std::cout << "Initialization from an existing string!" << std::endl;
+        wa = read4("stringData.mat", "wa", m, Lx, Ly, Lz);
+        wb = read4("stringData.mat", "wb", m, Lx, Ly, Lz);
+        phia = read4("stringData.mat", "phia", m, Lx, Ly, Lz);
+        phib = read4("stringData.mat", "phib", m, Lx, Ly, Lz);
+    } else {
+        std::cout << "Initializing string by two ends!" << std::endl;
+        const std::vector<double> a0 = read3("defect.mat", "wA", Lx, Ly, Lz), b0 = read3("defect.mat", "wB", Lx, Ly, Lz);
+        const std::vector<double> p0 = read3("defect.mat", "phiA", Lx, Ly, Lz), q0 = read3("defect.mat", "phiB", Lx, Ly, Lz);
+        const std::vector<double> a1 = read3("vLam.mat", "wA", Lx, Ly, Lz), b1 = read3("vLam.mat", "wB", Lx, Ly, Lz);
+        const std::vector<double> p1 = read3("vLam.mat", "phiA", Lx, Ly, Lz), q1 = read3("vLam.mat", "phiB", Lx, Ly, Lz);
+        wa.resize((size_t)m * M);
+        wb.resize(wa.size());
+        phia.resize(wa.size());
+        phib.resize(wa.size());
+        for (int s = 0; s < m; ++s) {
+            const double t = 1.0 * s / (m - 1);
+            for (size_t r = 0; r < M; ++r) {
+                wa[s * M + r] = a0[r] + t * (a1[r] - a0[r]);
+                wb[s * M + r] = b0[r] + t * (b1[r] - b0[r]);
+                phia[s * M + r] = p0[r] + t * (p1[r] - p0[r]);
+                phib[s * M + r] = q0[r] + t * (q1[r] - q0[r]);
+            }
+        }
+    }
+    // the ends' densities are restored after every redistribution (:319-328)
+    const std::vector<double> pa0(phia.begin(), phia.begin() + M), pb0(phib.begin(), phib.begin() + M);
+    const std::vector<double> pa1(phia.end() - M, phia.end()), pb1(phib.end() - M, phib.end());
+
+    po_string_desc d{};
+    d.grid = UnitCell(cfg).grid_desc(Lx, Ly, Lz);
+    const std::vector<unsigned> Ms = cfg.Ms();
+    const std::vector<double> lam = cfg.lam();
+    d.sA = (int)Ms.at(0);
+    d.sB = (int)Ms.at(1);
+    d.chiN = cfg.chiN().at(0);
+    d.lamA = lam.size() > 0 ? lam[0] : 0.1;
+    d.lamB = lam.size() > 1 ? lam[1] : 0.1;
+    d.lamY = lam.size() > 2 ? lam[2] : 1.0;
+    d.algo = cfg.algo_mde_type() == AlgorithmMDEType::OS2 ? PO_OS2 : PO_RQM4;
+    if (cfg.algo_mde_type() == AlgorithmMDEType::ETDRK4) {
+        std::cerr << "polystring: the string method runs RQM4 or OS2" << std::endl;
+        return 1;
+    }
+    d.contour = cfg.algo_contour_integration_type() == AlgorithmContourType::TRAPEZOIDAL ? PO_QUAD_TRAP : PO_QUAD_AUTO;
+    d.compressible = cfg.is_compressible() ? 1 : 0;
+    d.mirror = lower(cfg.get_string("Grid", "confine_mold")) == "nbc_by_pbc" ? 1 : 0;
+    d.m = m;
+    po_string *str = nullptr;
+    check(po_string_create(po_host_context(&cfg), &d, wa.data(), wb.data(), &str), "po_string_create");
+    check(po_string_set_sizes(str, stringSize.data()), "po_string_set_sizes");
+
+    // scft::run of each image: [Algorithm_SCFT] min_iter, max_iter, thresh_residual; [IO] record_interval
+    const int max_iter = (int)cfg.max_iter(), min_iter = (int)cfg.min_iter();
+    const int record_interval = std::max(1, (int)cfg.record_interval());
+    const double thresh = cfg.thresh_residual();
+    // records (maxT x m, column-major like the reference's blitz fortranArray output)
+    std::vector<double> H((size_t)maxT * m, 0.0), S((size_t)maxT * m, 0.0), dH((size_t)maxT * m, 10.0), F(maxT, 0.0);
+    std::vector<double> h(m), sv(m);
+    auto at = [maxT](int t, int s) { return (size_t)t + (size_t)maxT * s; };
+    auto save = [&]() {  // save_data (:224-254)
+        check(po_string_get(str, PO_FIELD_WA, wa.data()), "po_string_get");
+        check(po_string_get(str, PO_FIELD_WB, wb.data()), "po_string_get");
+        check(po_string_get(str, PO_FIELD_PHIA, phia.data()), "po_string_get");
+        check(po_string_get(str, PO_FIELD_PHIB, phib.data()), "po_string_get");
+        std::copy(pa0.begin(), pa0.end(), phia.begin());
+        std::copy(pb0.begin(), pb0.end(), phib.begin());
+        std::copy(pa1.begin(), pa1.end(), phia.end() - M);
+        std::copy(pb1.begin(), pb1.end(), phib.end() - M);
+        mat5::Writer w("result.mat");
+        if (!w.ok()) {
+            std::cerr << "polystring: cannot write result.mat" << std::endl;
+            std::exit(1);
+        }
+        const std::vector<int> d4 = {m, Lx, Ly, Lz};
+        w.put("wa", d4, to_fortran4(wa, m, Lx, Ly, Lz).data());
+        w.put("wb", d4, to_fortran4(wb, m, Lx, Ly, Lz).data());
+        w.put("phia", d4, to_fortran4(phia, m, Lx, Ly, Lz).data());
+        w.put("phib", d4, to_fortran4(phib, m, Lx, Ly, Lz).data());
+        w.put("F", {maxT, 1}, F.data());
+        w.put("H", {maxT, m}, H.data());
+        w.put("S", {maxT, m}, S.data());
+        w.put("dH", {maxT, m}, dH.data());
+    };
+    // the string loop (run_string, :130-150)
+    int st = 0;
+    double dmax = 0.0;
+    std::cout.precision(10);
+    do {
+        std::cout << "This is the " << st << "th string iteration!" << std::endl;
+        check(po_string_iterate(str, max_iter, min_iter, record_interval, thresh, h.data(), sv.data()),
+              "po_string_iterate");
+        double fsum = 0.0;
+        dmax = 0.0;
+        for (int s = 0; s < m; ++s) {
+            H[at(st, s)] = h[s];
+            S[at(st, s)] = sv[s];
+            fsum += h[s];
+            if (st > 0) dH[at(st, s)] = std::fabs(h[s] - H[at(st - 1, s)]);
+            dmax = std::max(dmax, dH[at(st, s)]);
+        }
+        F[st] = fsum / m;
+        std::cout << "F = " << F[st] << "  max dH = " << dmax << std::endl;
+        if (st % save_string_interval == 0) save();
+        st += 1;
+    } while (dmax > thresh_string_H && st < maxT);
+    std::cout << "string iterations: " << st << std::endl;
+    po_string_destroy(str);
+    return 0;
+}

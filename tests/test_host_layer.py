@@ -185,3 +185,88 @@ def test_cpp_model_batch_matches_models_alone(tmp_path):
     for l in lines[:5]:
         assert l[0] == l[1] == l[2], l
     assert lines[5][0] == lines[5][1]
+
+
+STRING_INI = """[IO]
+record_interval = 1
+[string]
+num_of_beads = 5
+string_iteration = 3
+save_string_interval = 1
+thresh_string_H = 1.0e-12
+is_readString = false
+is_changeSize = true
+size_S = 3.8
+size_E = 4.4
+[Model]
+model = AB
+chiN = [15]
+is_compressible = true
+[UnitCell]
+CrystalSystemType = RECT
+a = 4.0
+b = 4.4
+[Grid]
+dimension = 2
+Lx = 32
+Ly = 32
+Lz = 1
+confine_mold = NBC_by_PBC
+gridInitType = Data
+field_data = data.mat
+[Algorithm_MDE]
+algorithm = RQM4
+Ms = [11,11]
+[Algorithm_SCFT]
+algorithm =
+lam = [0.015, 0.015, 100.0]
+min_iter = 1
+max_iter = 3
+thresh_residual = 1.0e-10
+[Algorithm_Contour_Integration]
+algorithm = Simpson
+"""
+
+
+@pytest.mark.gpu
+def test_string_driver_matches_library(tmp_path, monkeypatch):
+    """polystring_AB_amd (the scft/polystring_AB.cc flow): paramString.ini,
+    defect.mat / vLam.mat ends, is_changeSize, 3 string iterations, result.mat
+    every iteration; checked bit for bit against the String binding on the
+    same inputs (launch shapes pinned in both processes: each autotunes, and
+    variants agree only to rounding)."""
+    from polystring_amd import capi
+    pins = dict(PO_SPLIT="0", PO_PLANE_VARIANT="0", PO_XPASS_VARIANT="0", PO_PLANE_HOLD="0")
+    for k, v in pins.items():
+        monkeypatch.setenv(k, v)
+    shape, m = (32, 32), 5
+    rng = np.random.default_rng(5)
+    ends = []
+    for name in ("defect.mat", "vLam.mat"):
+        wA = 7.5 + 2.0 * rng.standard_normal(shape) * 0.3
+        wB = 15.0 - wA
+        phiA = 0.5 + 0.1 * rng.standard_normal(shape)
+        sio.savemat(str(tmp_path / name), {"wA": wA, "wB": wB, "phiA": phiA, "phiB": 1.0 - phiA})
+        ends.append((wA, wB, phiA))
+    (tmp_path / "paramString.ini").write_text(STRING_INI)
+    print("\n[host] running polystring_AB_amd", flush=True)
+    out = subprocess.run([os.path.join(PKG, "polystring_AB_amd")], cwd=tmp_path, capture_output=True, text=True,
+                         timeout=120, env=dict(os.environ, **pins))
+    assert out.returncode == 0, out.stdout + out.stderr
+    r = sio.loadmat(str(tmp_path / "result.mat"))
+    wa = r["wa"].reshape((m,) + shape)
+    t = np.linspace(0.0, 1.0, m).reshape(m, 1, 1)
+    wa0 = ends[0][0] + t * (ends[1][0] - ends[0][0])
+    wb0 = ends[0][1] + t * (ends[1][1] - ends[0][1])
+    ctx = capi.Context(0, "fp64")
+    s = capi.String(ctx, wa0[..., None], wb0[..., None], cell=("RECTANGULAR", 4.0, 4.4), Ms=(11, 11), chiN=15.0,
+                    lam=(0.015, 0.015, 100.0), compressible=True, mirror=True, dim=2)
+    s.set_sizes(3.8 + np.arange(m) / (m - 1) * (4.4 - 3.8))  # size_S + s/(m-1) (size_E - size_S)
+    H = [s.iterate(3, 1, 1, 1e-10)[0] for _ in range(3)]
+    assert np.array_equal(r["H"], np.array(H))
+    assert np.allclose(r["F"].ravel(), np.mean(H, axis=1), rtol=0, atol=1e-15)
+    assert np.all(r["dH"][0] == 10.0) and np.array_equal(r["dH"][1:], np.abs(np.diff(H, axis=0)))
+    assert np.array_equal(wa, s.get("wA").reshape((m,) + shape))
+    phia = r["phia"].reshape((m,) + shape)
+    assert np.array_equal(phia[0], ends[0][2]) and np.array_equal(phia[-1], ends[1][2])
+    assert np.array_equal(phia[1:-1], s.get("phiA").reshape((m,) + shape)[1:-1])

@@ -141,6 +141,51 @@ template <bool INV, typename T> struct Dft<8, INV, T> {
     }
 };
 
+// Radix 11 (lengths 11 * 2^k, e.g. the 352-point grids of the reference's
+// string example), from the definition with the symmetric pairs
+// a_r = v_r + v_{11-r}, b_r = v_r - v_{11-r} (r = 1..5):
+//   X_k      = v_0 + sum_r a_r cos(2 pi r k / 11) -/+ i sum_r b_r sin(2 pi r k / 11),
+//   X_{11-k} = the same with the sine term's sign flipped (k = 1..5).
+template <bool INV, typename T> struct Dft<11, INV, T> {
+    static __device__ __forceinline__ void run(cx<T> *v) {
+        // cos and sin of 2 pi j / 11, j = 1..5 (40 digits)
+        const T C[6] = {(T)1, (T)0.8412535328311811688618116489193677175133,
+                        (T)0.4154150130018864255292741492296232035240, (T)-0.1423148382732851404437926686163696687911,
+                        (T)-0.6548607339452850640569250724662935531838, (T)-0.9594929736144973898903680570663276990625};
+        const T S[6] = {(T)0, (T)0.5406408174555975821076359543186916954318,
+                        (T)0.9096319953545183714117153830790284600602, (T)0.9898214418809327323760920377767187873765,
+                        (T)0.7557495743542582837740358439723444201797, (T)0.2817325568414296977114179153466168990358};
+        cx<T> a[5], b[5];
+#pragma unroll
+        for (int r = 1; r <= 5; ++r) {
+            a[r - 1] = add<T>(v[r], v[11 - r]);
+            b[r - 1] = sub<T>(v[r], v[11 - r]);
+        }
+        cx<T> x0 = v[0];
+#pragma unroll
+        for (int r = 0; r < 5; ++r) x0 = add<T>(x0, a[r]);
+        cx<T> out[11];
+        out[0] = x0;
+#pragma unroll
+        for (int k = 1; k <= 5; ++k) {
+            cx<T> re = v[0], im = mk<T>((T)0, (T)0);
+#pragma unroll
+            for (int r = 1; r <= 5; ++r) {
+                const int idx = (r * k) % 11;  // cos is even in idx -> 11 - idx, sin odd
+                const T c = idx <= 5 ? C[idx] : C[11 - idx];
+                const T sn = idx <= 5 ? S[idx] : -S[11 - idx];
+                re = add<T>(re, scl<T>(a[r - 1], c));
+                im = add<T>(im, scl<T>(b[r - 1], sn));
+            }
+            const cx<T> t = rot<INV, T>(im);  // forward -i sum b sin, inverse +i
+            out[k] = add<T>(re, t);
+            out[11 - k] = sub<T>(re, t);
+        }
+#pragma unroll
+        for (int k = 0; k < 11; ++k) v[k] = out[k];
+    }
+};
+
 // ---------------------------------------------------------- radix plans --
 // Radix sequences per supported length; the first stage has L = 1.
 template <int N> struct Plan;
@@ -159,6 +204,12 @@ template <> struct Plan<64> { static constexpr int n = 2; static constexpr int r
 template <> struct Plan<96> { static constexpr int n = 3; static constexpr int r[3] = {8, 4, 3}; };
 template <> struct Plan<128> { static constexpr int n = 3; static constexpr int r[3] = {8, 4, 4}; };
 template <> struct Plan<256> { static constexpr int n = 3; static constexpr int r[3] = {8, 8, 4}; };
+template <> struct Plan<11> { static constexpr int n = 1; static constexpr int r[1] = {11}; };
+template <> struct Plan<22> { static constexpr int n = 2; static constexpr int r[2] = {2, 11}; };
+template <> struct Plan<44> { static constexpr int n = 2; static constexpr int r[2] = {4, 11}; };
+template <> struct Plan<88> { static constexpr int n = 2; static constexpr int r[2] = {8, 11}; };
+template <> struct Plan<176> { static constexpr int n = 3; static constexpr int r[3] = {4, 4, 11}; };
+template <> struct Plan<352> { static constexpr int n = 3; static constexpr int r[3] = {8, 4, 11}; };
 
 // One Stockham stage over NP pencils.  PFAST: consecutive threads take
 // consecutive pencils (use when pencils are adjacent in LDS); otherwise

@@ -1167,7 +1167,7 @@ k_etd_gather3(const T *__restrict__ src, T *__restrict__ dst, int ncol) {
 
 // ------------------------------------------------------------- dispatch --
 #define PO_PLANE_SHAPES(X) \
-    X(1, 1) X(8, 1) X(16, 1) X(32, 1) X(48, 1) X(64, 1) X(96, 1) X(128, 1) X(256, 1) \
+    X(1, 1) X(8, 1) X(16, 1) X(32, 1) X(48, 1) X(64, 1) X(96, 1) X(128, 1) X(256, 1) X(88, 1) X(176, 1) X(352, 1) \
     X(8, 8) X(16, 16) X(32, 32) X(48, 48) X(64, 64) X(96, 96) X(128, 128)
 #define PO_X_SIZES(X) X(8) X(16) X(32) X(48) X(64) X(96) X(128) X(256)
 

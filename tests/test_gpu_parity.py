@@ -32,6 +32,8 @@ def relerr(a, b):
 
 
 CASES = [
+    ((32, 88, 1), ("RECTANGULAR", 4.0, 9.5), 2),             # radix-11 y transform (88 = 8 x 11)
+    ((128, 352, 1), ("RECTANGULAR", 26.3145, 70.1720), 2),   # the reference's string grid and cell (352 = 8 x 4 x 11)
     ((32, 32, 32), ("CUBIC", 4.0), 3),
     ((64, 1, 1), ("LAMELLAR", 4.0), 1),
     ((32, 32, 1), ("SQUARE", 4.0), 2),
@@ -139,6 +141,21 @@ def test_model_iterations_match_oracle(ctx64, algo):
         gm.update()
         print(f"  it {it}: H gpu {gm.observe()['H']!r} oracle {om.H()!r}", flush=True)
         _compare_models(gm, om, 1e-10)
+
+
+def test_radix11_model_iterations_match_oracle(ctx64):
+    """2-D grid with a 176-point (4 x 4 x 11) y transform: 4 Anderson
+    iterations against the oracle to 1e-10."""
+    shape = (32, 176, 1)
+    wA = 10.0 + smooth_field(shape, 21)
+    wB = 10.0 - smooth_field(shape, 22)
+    kw = dict(cell=("RECTANGULAR", 4.0, 18.0), Ms=(11, 21), chiN=20.0, lam=(0.1, 0.1), n_anderson=3)
+    om = O.Model(wA, wB, dim=2, **kw)
+    gm = capi.ModelAB(ctx64, wA, wB, dim=2, **kw)
+    for _ in range(4):
+        om.update()
+        gm.update()
+    _compare_models(gm, om, 1e-10)
 
 
 def test_quad4_branch_matches_oracle(ctx64):

@@ -149,3 +149,21 @@ def test_string_per_image_sizes_match_oracle(ctx64, shape, cellf, dim, Ms, compr
     assert relerr(Hb, Ho) > 1e-6
     with pytest.raises(capi.POError):
         g.set_sizes(np.full(m, -1.0))
+
+
+def test_reference_string_grid_reproduces_endpoint_record(ctx64):
+    """The reference's string example grid and model (string/paramString.ini:
+    2-D 128 x 352, RECT 26.3145 x 70.1720, compressible, chiN = 30, Ms =
+    [21, 21], lam = [0.015, 0.015, 100], NBC_by_PBC) on the GPU: a string of
+    three copies of the converged lamellar end state (string/vLam.mat), one EM
+    update per image, gives each image the reference's own recorded
+    F = 4.688474722862775 for that state."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref_string_vlam.npz"))
+    w = np.stack([g["wA"].reshape(128, 352, 1)] * 3)
+    v = np.stack([g["wB"].reshape(128, 352, 1)] * 3)
+    s = capi.String(ctx64, w, v, cell=("RECTANGULAR", 26.3145, 70.1720), Ms=(21, 21), chiN=30.0,
+                    lam=(0.015, 0.015, 100.0), compressible=True, mirror=True, dim=2)
+    H, S = s.iterate(1, 1, 1, 0.0)
+    print(f"\n[ref string grid] H {H.tolist()} recorded {float(g['F'])!r}", flush=True)
+    assert np.abs(H - float(g["F"])).max() <= 1e-11 * float(g["F"])

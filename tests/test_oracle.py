@@ -381,3 +381,22 @@ def test_unnormalised_phi_keeps_symmetric_schemes():
             break
     assert m.residual() < 1e-10
     assert abs(m.H() - known["RQM4+Anderson"]["H"]) < 1e-8
+
+
+def test_reference_string_endpoint_record():
+    """The reference's string example (string/paramString.ini, vLam.mat):
+    compressible AB melt, chiN = 30, Ms = [21, 21], lam = [0.015, 0.015, 100],
+    2-D 128 x 352 grid (352 = 32 x 11: a radix-11 transform), RECT cell
+    26.3145 x 70.1720, NBC_by_PBC mirroring.  One EM update of the oracle from
+    the stored converged fields reproduces the reference's own record for
+    them (last of 20000 iterations): F = 4.688474722862775 (measured
+    agreement 1.3e-13) and the residual 7.754e-8 (the record one iteration
+    later; 7.7529e-8 measured)."""
+    g = np.load(os.path.join(GOLD, "ref_string_vlam.npz"))
+    wA, wB = g["wA"].reshape(128, 352, 1), g["wB"].reshape(128, 352, 1)
+    om = O.EMModel(wA, wB, dim=2, cell=("RECTANGULAR", 26.3145, 70.1720), Ms=(21, 21), chiN=30.0,
+                   lam=(0.015, 0.015, 100.0), mirror=True, compressible=True)
+    om.update()
+    assert abs(om.H() - float(g["F"])) <= 1e-11 * float(g["F"])
+    assert abs(om.residual() - float(g["err_residual"])) <= 1e-3 * float(g["err_residual"])
+    assert abs(om.Q() - float(g["Q"])) <= 1e-12 * float(g["Q"])

@@ -161,3 +161,30 @@ def test_batch_members_refuse_destroy(ctx64):
     b.close()
     assert lib.po_model_destroy(models[0].h) == capi.PO_OK
     models[0].h = None
+
+
+def test_batch_general_cell_os2(ctx64, monkeypatch):
+    """OS2 on hexagonal cells (full 27-image k^2 tables, no split layout):
+    a batch of two systems with different cells gives each the same bits as
+    updating it alone; the fused adjoint is refused (it needs separable
+    tables)."""
+    for k in PINS:
+        monkeypatch.setenv(k, "0")
+    shape = (32, 32, 32)
+    alone, batched = [], []
+    for i, c in enumerate((3.5, 3.9)):
+        wA = smooth_field(shape, 81 + i, offset=2.0)
+        wB = smooth_field(shape, 91 + i, offset=-1.0)
+        kw = dict(cell=("HEXAGONAL", 4.0, 4.0, c), Ms=(9, 13), chiN=14.0, algo="OS2", n_anderson=3)
+        alone.append(capi.ModelAB(ctx64, wA, wB, **kw))
+        batched.append(capi.ModelAB(ctx64, wA, wB, **kw))
+    with pytest.raises(capi.POError):
+        capi.Batch(batched, adjoint=True)
+    b = capi.Batch(batched)
+    for _ in range(3):
+        b.update()
+        for a, m in zip(alone, batched):
+            a.update()
+            assert a.observe() == m.observe()
+    for a, m in zip(alone, batched):
+        assert np.array_equal(a.get("wA"), m.get("wA"))

@@ -30,6 +30,9 @@ def test_binding_loads_and_reports_version():
     assert lib.po_shape_supported(128, 128, 128) == 1
     assert lib.po_shape_supported(64, 1, 1) == 1
     assert lib.po_shape_supported(100, 100, 100) == 0
+    # radix-11 y transforms of 2-D grids, among them the reference's string grid 128 x 352
+    assert all(lib.po_shape_supported(128, ly, 1) == 1 for ly in (88, 176, 352))
+    assert lib.po_shape_supported(128, 352, 352) == 0 and lib.po_shape_supported(352, 32, 1) == 0
 
 
 def test_struct_layouts_match_header():

@@ -277,6 +277,16 @@ int po_batch_destroy(po_batch *b);
 int po_batch_update(po_batch *b, int n_iter);
 /* contour steps of one batched update (2 Ns per system) */
 long po_batch_contour_steps(const po_batch *b);
+/* po_models_relax over the batch's models (the scft::relax rules per model,
+ * src/scft.cc:382-422,536-568): every active model takes the same batched
+ * updates; a model that stops leaves the shared program, keeping the state of
+ * its last update.  A batched cell or parameter scan on one grid: the
+ * counterpart of the reference's MANUAL / BATCH cell scan (scft::run,
+ * src/scft.cc:82-95), which relaxes one size after another, each starting
+ * from the previous size's fields; here every model starts from its own
+ * fields.  iters, H, residual (optional) get n values. */
+int po_batch_relax(po_batch *b, int max_iter, int min_iter, int record_interval, double thresh, int *iters,
+                   double *H, double *residual);
 /* the po_model_set_profiling / po_model_profile hooks of the shared program */
 int po_batch_set_profiling(po_batch *b, int on);
 int po_batch_profile(po_batch *b, double *out4);

@@ -48,6 +48,7 @@ EXPORTS = [
     "po_model_set_profiling", "po_model_profile", "po_model_launch_shape",
     "po_batch_create", "po_batch_destroy", "po_batch_update", "po_batch_contour_steps",
     "po_batch_set_profiling", "po_batch_profile", "po_batch_launch_shape", "po_string_set_sizes",
+    "po_batch_relax",
 ]
 
 
@@ -121,6 +122,7 @@ def load():
         "po_batch_set_profiling": ([_vp, i], i), "po_batch_profile": ([_vp, _dp], i),
         "po_batch_launch_shape": ([_vp, ctypes.POINTER(ctypes.c_int)], i),
         "po_string_set_sizes": ([_vp, _dp], i),
+        "po_batch_relax": ([_vp, i, i, i, d, ctypes.POINTER(i), _dp, _dp], i),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -379,6 +381,16 @@ class Batch:
 
     def contour_steps(self):
         return load().po_batch_contour_steps(self.h)
+
+    def relax(self, max_iter, min_iter=10, record_interval=1, thresh=1e-8):
+        """po_batch_relax: each model relaxed by the scft::relax rules; returns (iters, H, residual)."""
+        n = len(self.models)
+        it = (ctypes.c_int * n)()
+        H = np.zeros(n)
+        r = np.zeros(n)
+        _check(load().po_batch_relax(self.h, int(max_iter), int(min_iter), int(record_interval), float(thresh), it,
+                                     _p(H), _p(r)))
+        return np.array(list(it)), H, r
 
     def set_profiling(self, on=True):
         _check(load().po_batch_set_profiling(self.h, 1 if on else 0))

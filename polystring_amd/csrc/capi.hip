@@ -928,13 +928,17 @@ struct po_batch {
         return hipEventRecord(ev[ev_used++], ctx->stream);
     }
     virtual int update(int n) = 0;
+    virtual int relax(int max_iter, int min_iter, int record_interval, double thresh, int *iters, double *H,
+                      double *residual) = 0;
     virtual void launch_shape(int *out5) = 0;
 };
 
 namespace po {
 
 template <typename T> struct Batch : po_batch {
-    std::vector<Model<T> *> ms;
+    std::vector<Model<T> *> all;  // every model of the batch
+    std::vector<Model<T> *> ms;   // the models the program advances (all, or the active ones in relax)
+    int split = 0, xv = 0, pv = 0, hold = 0;  // the launch shape, fixed at creation
     Program<T> prog;
     bool adjoint = false;
     bool dual = true;         // forward and complementary chains on two streams
@@ -966,21 +970,13 @@ template <typename T> struct Batch : po_batch {
         // the first model's tuned shape; the fused adjoint takes the
         // 16-column x tiles unless PO_XPASS_VARIANT pins the tile (the same
         // rule as a single model's fused program)
-        const int split = m0.prog.split;
-        const int xv = (adj && !std::getenv("PO_XPASS_VARIANT")) ? 0 : m0.prog.x_variant;
-        std::vector<Lane<T>> lanes;
-        for (int role = 0; role < 2; ++role)
-            for (auto *m : ms) lanes.push_back(m->lanes_[role]);
-        std::vector<AdjointArena<T>> arenas(adj ? n : 0);
+        split = m0.prog.split;
+        xv = (adj && !std::getenv("PO_XPASS_VARIANT")) ? 0 : m0.prog.x_variant;
+        pv = m0.prog.plane_variant;
+        hold = m0.prog.hold;
+        all = ms;
         int rc;
-        for (int i = 0; i < (adj ? n : 0); ++i)
-            if ((rc = ms[i]->fused_arena(xv, split, arenas[i]))) return rc;
-        prog.slice_elems = m0.sh.M;
-        if ((rc = prog.build(m0.desc.algo, lanes, ctx->stream, adj ? arenas.data() : nullptr, n))) return rc;
-        prog.plane_variant = m0.prog.plane_variant;
-        prog.x_variant = xv;
-        prog.split = split;
-        prog.hold = m0.prog.hold;
+        if ((rc = build_program(all))) return rc;
         steps = m0.steps * n;
         lockstep = m0.steps / 2;
         // PO_DUAL=0: every lane of a launch on the context's stream
@@ -989,8 +985,87 @@ template <typename T> struct Batch : po_batch {
         bool ok = hipStreamCreateWithFlags(&st2, hipStreamNonBlocking) == hipSuccess;
         for (auto &e : dev) ok = ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
         if (!ok) return fail(PO_ERR_DEVICE, "batch stream / event creation failed");
-        for (auto *m : ms) ++m->in_batch;
+        for (auto *m : all) ++m->in_batch;
         registered = true;
+        return PO_OK;
+    }
+
+    // The shared launch program over the models in sub (lanes: their forward
+    // chains, then their complementary chains; with the adjoint one arena
+    // per model), in the batch's launch shape.
+    int build_program(const std::vector<Model<T> *> &sub) {
+        std::vector<Lane<T>> lanes;
+        for (int role = 0; role < 2; ++role)
+            for (auto *m : sub) lanes.push_back(m->lanes_[role]);
+        std::vector<AdjointArena<T>> arenas(adjoint ? sub.size() : 0);
+        int rc;
+        for (size_t i = 0; i < arenas.size(); ++i)
+            if ((rc = sub[i]->fused_arena(xv, split, arenas[i]))) return rc;
+        prog.reset();
+        prog.slice_elems = sub[0]->sh.M;
+        if ((rc = prog.build(sub[0]->desc.algo, lanes, ctx->stream, adjoint ? arenas.data() : nullptr,
+                             (int)sub.size())))
+            return rc;
+        prog.plane_variant = pv;
+        prog.x_variant = xv;
+        prog.split = split;
+        prog.hold = hold;
+        ms = sub;
+        return PO_OK;
+    }
+
+    // po_models_relax for the batch: the scft::relax rules per model
+    // (src/scft.cc:382-422,536-568).  All active models share the iteration
+    // count; a model that stops leaves the program, which is rebuilt over the
+    // rest, so it keeps exactly the state of its last update.
+    int relax(int max_iter, int min_iter, int record_interval, double thresh, int *iters, double *H,
+              double *residual) override {
+        const int n = (int)all.size();
+        std::vector<int> it(n, 0), nrec(n, 0), act(n, 1);
+        std::vector<double> h(n, 0.0), r(n, 0.0);
+        int rc, left = n, cur = 0;
+        if (ms.size() != all.size() && (rc = build_program(all))) return rc;
+        while (left > 0) {
+            const int chunk = std::min(record_interval - cur % record_interval, max_iter - cur);
+            if ((rc = update(chunk))) return rc;
+            cur += chunk;
+            bool changed = false;
+            for (int i = 0; i < n; ++i) {
+                if (!act[i]) continue;
+                it[i] = cur;
+                bool stop = cur >= max_iter;
+                if (cur % record_interval == 0) {
+                    double o[5];
+                    if ((rc = all[i]->observe(o))) return rc;
+                    ++nrec[i];
+                    h[i] = o[0];
+                    r[i] = o[3];
+                    if (!std::isfinite(o[0])) stop = true;                             // src/scft.cc:537-543
+                    if (cur > min_iter && nrec[i] > 1 && o[3] < thresh) stop = true;  // src/scft.cc:560-568
+                }
+                if (stop) {
+                    act[i] = 0;
+                    --left;
+                    changed = true;
+                }
+            }
+            if (changed && left > 0) {
+                std::vector<Model<T> *> sub;
+                for (int i = 0; i < n; ++i)
+                    if (act[i]) sub.push_back(all[i]);
+                PO_HIP(hipStreamSynchronize(ctx->stream));
+                if ((rc = build_program(sub))) return rc;
+            }
+        }
+        if (ms.size() != all.size()) {
+            PO_HIP(hipStreamSynchronize(ctx->stream));
+            if ((rc = build_program(all))) return rc;
+        }
+        for (int i = 0; i < n; ++i) {
+            if (iters) iters[i] = it[i];
+            if (H) H[i] = h[i];
+            if (residual) residual[i] = r[i];
+        }
         return PO_OK;
     }
 
@@ -1034,7 +1109,7 @@ template <typename T> struct Batch : po_batch {
 
     ~Batch() override {
         if (registered)
-            for (auto *m : ms) --m->in_batch;
+            for (auto *m : all) --m->in_batch;
         if (st2) {
             (void)hipStreamSynchronize(st2);
             (void)hipStreamDestroy(st2);
@@ -1503,6 +1578,13 @@ int po_batch_update(po_batch *b, int n_iter) {
 }
 
 long po_batch_contour_steps(const po_batch *b) { return b ? b->steps : -1; }
+
+int po_batch_relax(po_batch *b, int max_iter, int min_iter, int record_interval, double thresh, int *iters,
+                   double *H, double *residual) {
+    if (!b || max_iter < 1 || record_interval < 1) return fail(PO_ERR_INVALID, "bad argument");
+    PO_HIP(hipSetDevice(b->ctx->device));
+    return b->relax(max_iter, min_iter, record_interval, thresh, iters, H, residual);
+}
 
 int po_batch_set_profiling(po_batch *b, int on) {
     if (!b) return fail(PO_ERR_INVALID, "null batch");

@@ -188,3 +188,33 @@ def test_batch_general_cell_os2(ctx64, monkeypatch):
             assert a.observe() == m.observe()
     for a, m in zip(alone, batched):
         assert np.array_equal(a.get("wA"), m.get("wA"))
+
+
+def test_batch_relax_matches_models_relax(ctx64, monkeypatch):
+    """po_batch_relax (a batched cell scan: the published 1-D LAM field at
+    three cell sizes) gives every model the iterations, H and residual that
+    po_models_relax gives the same models alone, bit for bit; models leave
+    the shared program as they converge, and the batch then updates all of
+    them again."""
+    import json
+    import os
+    for k in PINS:
+        monkeypatch.setenv(k, "0")
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    f = np.load(os.path.join(gold, "lam1d_field.npz"))
+    kws = [dict(cell=("LAMELLAR", a), Ms=(51, 51), chiN=30.0, lam=(0.05, 0.05), n_anderson=5, dim=1)
+           for a in (3.9, 4.0, 4.45)]
+    alone = [capi.ModelAB(ctx64, f["wA"], f["wB"], **kw) for kw in kws]
+    batched = [capi.ModelAB(ctx64, f["wA"], f["wB"], **kw) for kw in kws]
+    it0, H0, r0 = capi.relax_models(alone, 2000, 10, 1, 1e-9)
+    b = capi.Batch(batched)
+    it1, H1, r1 = b.relax(2000, 10, 1, 1e-9)
+    print(f"\n[batch relax] iters {it1.tolist()} H {H1.tolist()}", flush=True)
+    assert np.array_equal(it0, it1) and np.array_equal(H0, H1) and np.array_equal(r0, r1)
+    assert len(set(it1.tolist())) > 1  # the sizes converge at different iterations
+    Hpub = json.load(open(os.path.join(gold, "known_answers.json")))["1D"]["RQM4+Anderson"]["H"]
+    assert abs(H1[1] - Hpub) <= 1e-8 * Hpub  # a = 4.0: the published free energy
+    b.update(1)
+    for a, m in zip(alone, batched):
+        a.update(1)
+        assert a.observe() == m.observe()

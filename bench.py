@@ -185,8 +185,11 @@ def run_string(iters=2, precision="fp32"):
 
 def run_scan(iters=5, precision="fp32"):
     """Ragged scan: 24 independent solves of uneven grid (32..96^3) and
-    contour count (Ns 100..200), each on its own stream, propagators replayed
-    from hipGraphs, relaxed together by po_models_relax."""
+    contour count (Ns 100..200), `iters` updates each.  Timed twice: every
+    solve on its own stream with its propagators replayed from a hipGraph
+    (po_models_relax), and with the solves of one grid and Ns batched into
+    one launch program per group (po_batch_*), the groups on their own
+    contexts.  The batched form is the reported value."""
     from polystring_amd import capi, workloads
     wls = workloads.cfg5b()
     ctxs, models = [], []
@@ -201,9 +204,42 @@ def run_scan(iters=5, precision="fp32"):
     it, H, r = capi.relax_models(models, iters, iters, iters, 0.0)
     wall = time.perf_counter() - t0
     steps = sum(int(n) * m.contour_steps() for n, m in zip(it, models))
-    out = dict(workload=f"ragged scan, {len(wls)} solves, grids 32-96^3, Ns 100-200, {precision}",
-               iterations_each=iters, contour_steps_per_s=steps / wall, solves=len(wls))
+    per_model = steps / wall
     del models
+    for c in ctxs:
+        c.close()
+    # the same solves with those of one grid and contour count batched
+    # (po_batch_*), one context per group so the groups overlap on the device
+    import collections
+    groups = collections.defaultdict(list)
+    for w in wls:
+        groups[(w.shape, w.Ns)].append(w)
+    ctxs, runners, models = [], [], []
+    for ws in groups.values():
+        c = capi.Context(0, precision)
+        ms = [capi.ModelAB(c, w.wA, w.wB, **w.model_kwargs()) for w in ws]
+        models += ms
+        ctxs.append(c)
+        runners.append(capi.Batch(ms) if len(ms) > 1 else ms[0])
+    for r in runners:  # warm-up
+        r.update(1)
+    for c in ctxs:
+        c.synchronize()
+    t0 = time.perf_counter()
+    for r in runners:  # queued without host synchronisation: the groups overlap
+        r.update(iters)
+    for c in ctxs:
+        c.synchronize()
+    wall = time.perf_counter() - t0
+    steps = sum(iters * m.contour_steps() for m in models)
+    out = dict(workload=f"ragged scan, {len(wls)} solves, grids 32-96^3, Ns 100-200, {precision}; solves of one "
+                        f"grid and Ns batched ({len(runners)} groups, one context each)",
+               iterations_each=iters, contour_steps_per_s=steps / wall, solves=len(wls),
+               contour_steps_per_s_per_model_streams=per_model)
+    for r in runners:
+        if isinstance(r, capi.Batch):
+            r.close()
+    del runners, models
     for c in ctxs:
         c.close()
     return out

@@ -188,8 +188,8 @@ def run_scan(iters=5, precision="fp32"):
     contour count (Ns 100..200), `iters` updates each.  Timed twice: every
     solve on its own stream with its propagators replayed from a hipGraph
     (po_models_relax), and with the solves of one grid and Ns batched into
-    one launch program per group (po_batch_*), the groups on their own
-    contexts.  The batched form is the reported value."""
+    one launch program per group, the groups on their own contexts, relaxed
+    round-robin by po_batches_relax.  The batched form is the reported value."""
     from polystring_amd import capi, workloads
     wls = workloads.cfg5b()
     ctxs, models = [], []
@@ -214,32 +214,25 @@ def run_scan(iters=5, precision="fp32"):
     groups = collections.defaultdict(list)
     for w in wls:
         groups[(w.shape, w.Ns)].append(w)
-    ctxs, runners, models = [], [], []
+    ctxs, batches, models = [], [], []
     for ws in groups.values():
         c = capi.Context(0, precision)
         ms = [capi.ModelAB(c, w.wA, w.wB, **w.model_kwargs()) for w in ws]
         models += ms
         ctxs.append(c)
-        runners.append(capi.Batch(ms) if len(ms) > 1 else ms[0])
-    for r in runners:  # warm-up
-        r.update(1)
-    for c in ctxs:
-        c.synchronize()
+        batches.append(capi.Batch(ms))
+    capi.relax_batches(batches, 1, 1, 1, 0.0)  # warm-up
     t0 = time.perf_counter()
-    for r in runners:  # queued without host synchronisation: the groups overlap
-        r.update(iters)
-    for c in ctxs:
-        c.synchronize()
+    it, H, r = capi.relax_batches(batches, iters, iters, iters, 0.0)  # round-robin over the groups
     wall = time.perf_counter() - t0
-    steps = sum(iters * m.contour_steps() for m in models)
+    steps = sum(int(n) * m.contour_steps() for n, m in zip(it, models))
     out = dict(workload=f"ragged scan, {len(wls)} solves, grids 32-96^3, Ns 100-200, {precision}; solves of one "
-                        f"grid and Ns batched ({len(runners)} groups, one context each)",
+                        f"grid and Ns batched ({len(batches)} groups, one context each, po_batches_relax)",
                iterations_each=iters, contour_steps_per_s=steps / wall, solves=len(wls),
                contour_steps_per_s_per_model_streams=per_model)
-    for r in runners:
-        if isinstance(r, capi.Batch):
-            r.close()
-    del runners, models
+    for b in batches:
+        b.close()
+    del batches, models
     for c in ctxs:
         c.close()
     return out

@@ -287,6 +287,14 @@ long po_batch_contour_steps(const po_batch *b);
  * fields.  iters, H, residual (optional) get n values. */
 int po_batch_relax(po_batch *b, int max_iter, int min_iter, int record_interval, double thresh, int *iters,
                    double *H, double *residual);
+/* po_batch_relax over nb batches at once (a ragged scan: one batch per grid
+ * and contour length, each on its own context).  Round-robin as
+ * po_models_relax: every active batch's next chunk of updates is queued
+ * before any is observed, so batches on different contexts overlap on the
+ * device.  iters, H, residual (optional) get the models of batch 0, then of
+ * batch 1, and so on. */
+int po_batches_relax(po_batch **batches, int nb, int max_iter, int min_iter, int record_interval, double thresh,
+                     int *iters, double *H, double *residual);
 /* the po_model_set_profiling / po_model_profile hooks of the shared program */
 int po_batch_set_profiling(po_batch *b, int on);
 int po_batch_profile(po_batch *b, double *out4);

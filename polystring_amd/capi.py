@@ -48,7 +48,7 @@ EXPORTS = [
     "po_model_set_profiling", "po_model_profile", "po_model_launch_shape",
     "po_batch_create", "po_batch_destroy", "po_batch_update", "po_batch_contour_steps",
     "po_batch_set_profiling", "po_batch_profile", "po_batch_launch_shape", "po_string_set_sizes",
-    "po_batch_relax",
+    "po_batch_relax", "po_batches_relax",
 ]
 
 
@@ -123,6 +123,7 @@ def load():
         "po_batch_launch_shape": ([_vp, ctypes.POINTER(ctypes.c_int)], i),
         "po_string_set_sizes": ([_vp, _dp], i),
         "po_batch_relax": ([_vp, i, i, i, d, ctypes.POINTER(i), _dp, _dp], i),
+        "po_batches_relax": ([ctypes.POINTER(_vp), i, i, i, i, d, ctypes.POINTER(i), _dp, _dp], i),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -404,6 +405,21 @@ class Batch:
         o = (ctypes.c_int * 5)()
         _check(load().po_batch_launch_shape(self.h, o))
         return dict(PO_SPLIT=o[0], PO_PLANE_VARIANT=o[1], PO_XPASS_VARIANT=o[2], PO_PLANE_HOLD=o[3], PO_DUAL=o[4])
+
+
+def relax_batches(batches, max_iter, min_iter=10, record_interval=1, thresh=1e-8):
+    """po_batches_relax: several batches relaxed round-robin (per-model
+    scft::relax rules); returns (iters, H, residual) over the models of batch
+    0, then batch 1, ..."""
+    nb = len(batches)
+    n = sum(len(b.models) for b in batches)
+    hs = (_vp * nb)(*[b.h for b in batches])
+    it = (ctypes.c_int * n)()
+    H = np.zeros(n)
+    r = np.zeros(n)
+    _check(load().po_batches_relax(hs, nb, int(max_iter), int(min_iter), int(record_interval), float(thresh), it,
+                                   _p(H), _p(r)))
+    return np.array(list(it)), H, r
 
 
 class String:

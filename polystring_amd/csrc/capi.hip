@@ -928,8 +928,12 @@ struct po_batch {
         return hipEventRecord(ev[ev_used++], ctx->stream);
     }
     virtual int update(int n) = 0;
-    virtual int relax(int max_iter, int min_iter, int record_interval, double thresh, int *iters, double *H,
-                      double *residual) = 0;
+    virtual int relax_begin() = 0;
+    virtual bool relax_active() const = 0;
+    virtual int relax_issue(int max_iter, int record_interval) = 0;
+    virtual int relax_collect(int max_iter, int min_iter, int record_interval, double thresh) = 0;
+    virtual void relax_result(int *iters, double *H, double *residual) const = 0;
+    virtual int size() const = 0;
     virtual void launch_shape(int *out5) = 0;
 };
 
@@ -1017,57 +1021,83 @@ template <typename T> struct Batch : po_batch {
     // po_models_relax for the batch: the scft::relax rules per model
     // (src/scft.cc:382-422,536-568).  All active models share the iteration
     // count; a model that stops leaves the program, which is rebuilt over the
-    // rest, so it keeps exactly the state of its last update.
-    int relax(int max_iter, int min_iter, int record_interval, double thresh, int *iters, double *H,
-              double *residual) override {
+    // rest, so it keeps exactly the state of its last update.  Split into
+    // begin / issue (queues one chunk of updates, no host synchronisation) /
+    // collect (observes, stops, rebuilds) so that several batches can
+    // advance round-robin (po_batches_relax).
+    std::vector<int> r_it, r_nrec, r_act;
+    std::vector<double> r_h, r_r;
+    int r_cur = 0, r_left = 0, r_chunk = 0;
+
+    int relax_begin() override {
         const int n = (int)all.size();
-        std::vector<int> it(n, 0), nrec(n, 0), act(n, 1);
-        std::vector<double> h(n, 0.0), r(n, 0.0);
-        int rc, left = n, cur = 0;
-        if (ms.size() != all.size() && (rc = build_program(all))) return rc;
-        while (left > 0) {
-            const int chunk = std::min(record_interval - cur % record_interval, max_iter - cur);
-            if ((rc = update(chunk))) return rc;
-            cur += chunk;
-            bool changed = false;
-            for (int i = 0; i < n; ++i) {
-                if (!act[i]) continue;
-                it[i] = cur;
-                bool stop = cur >= max_iter;
-                if (cur % record_interval == 0) {
-                    double o[5];
-                    if ((rc = all[i]->observe(o))) return rc;
-                    ++nrec[i];
-                    h[i] = o[0];
-                    r[i] = o[3];
-                    if (!std::isfinite(o[0])) stop = true;                             // src/scft.cc:537-543
-                    if (cur > min_iter && nrec[i] > 1 && o[3] < thresh) stop = true;  // src/scft.cc:560-568
-                }
-                if (stop) {
-                    act[i] = 0;
-                    --left;
-                    changed = true;
-                }
-            }
-            if (changed && left > 0) {
-                std::vector<Model<T> *> sub;
-                for (int i = 0; i < n; ++i)
-                    if (act[i]) sub.push_back(all[i]);
-                PO_HIP(hipStreamSynchronize(ctx->stream));
-                if ((rc = build_program(sub))) return rc;
-            }
-        }
+        r_it.assign(n, 0);
+        r_nrec.assign(n, 0);
+        r_act.assign(n, 1);
+        r_h.assign(n, 0.0);
+        r_r.assign(n, 0.0);
+        r_cur = 0;
+        r_left = n;
         if (ms.size() != all.size()) {
             PO_HIP(hipStreamSynchronize(ctx->stream));
-            if ((rc = build_program(all))) return rc;
-        }
-        for (int i = 0; i < n; ++i) {
-            if (iters) iters[i] = it[i];
-            if (H) H[i] = h[i];
-            if (residual) residual[i] = r[i];
+            return build_program(all);
         }
         return PO_OK;
     }
+
+    bool relax_active() const override { return r_left > 0; }
+
+    int relax_issue(int max_iter, int record_interval) override {
+        if (r_left == 0) return PO_OK;
+        r_chunk = std::min(record_interval - r_cur % record_interval, max_iter - r_cur);
+        return update(r_chunk);
+    }
+
+    int relax_collect(int max_iter, int min_iter, int record_interval, double thresh) override {
+        if (r_left == 0) return PO_OK;
+        r_cur += r_chunk;
+        const int n = (int)all.size();
+        bool changed = false;
+        int rc;
+        for (int i = 0; i < n; ++i) {
+            if (!r_act[i]) continue;
+            r_it[i] = r_cur;
+            bool stop = r_cur >= max_iter;
+            if (r_cur % record_interval == 0) {
+                double o[5];
+                if ((rc = all[i]->observe(o))) return rc;
+                ++r_nrec[i];
+                r_h[i] = o[0];
+                r_r[i] = o[3];
+                if (!std::isfinite(o[0])) stop = true;                                 // src/scft.cc:537-543
+                if (r_cur > min_iter && r_nrec[i] > 1 && o[3] < thresh) stop = true;  // src/scft.cc:560-568
+            }
+            if (stop) {
+                r_act[i] = 0;
+                --r_left;
+                changed = true;
+            }
+        }
+        if (changed) {
+            std::vector<Model<T> *> sub;
+            for (int i = 0; i < n; ++i)
+                if (r_act[i]) sub.push_back(all[i]);
+            PO_HIP(hipStreamSynchronize(ctx->stream));
+            // the last model out restores the program over every model
+            if ((rc = build_program(sub.empty() ? all : sub))) return rc;
+        }
+        return PO_OK;
+    }
+
+    void relax_result(int *iters, double *H, double *residual) const override {
+        for (size_t i = 0; i < all.size(); ++i) {
+            if (iters) iters[i] = r_it[i];
+            if (H) H[i] = r_h[i];
+            if (residual) residual[i] = r_r[i];
+        }
+    }
+
+    int size() const override { return (int)all.size(); }
 
     // One Model_AB::update of every system: the shared program with the
     // forward chains on the context's stream and the complementary chains on
@@ -1579,11 +1609,46 @@ int po_batch_update(po_batch *b, int n_iter) {
 
 long po_batch_contour_steps(const po_batch *b) { return b ? b->steps : -1; }
 
+int po_batches_relax(po_batch **bs, int nb, int max_iter, int min_iter, int record_interval, double thresh,
+                     int *iters, double *H, double *residual) {
+    if (!bs || nb < 1 || max_iter < 1 || record_interval < 1) return fail(PO_ERR_INVALID, "bad argument");
+    for (int i = 0; i < nb; ++i) {
+        if (!bs[i]) return fail(PO_ERR_INVALID, "null batch in the list");
+        for (int j = 0; j < i; ++j)
+            if (bs[j] == bs[i]) return fail(PO_ERR_INVALID, "a batch appears twice in the list");
+    }
+    int rc;
+    for (int i = 0; i < nb; ++i) {
+        PO_HIP(hipSetDevice(bs[i]->ctx->device));
+        if ((rc = bs[i]->relax_begin())) return rc;
+    }
+    // round-robin: every batch's chunk is queued before any is observed, so
+    // batches on different contexts overlap on the device
+    for (bool any = true; any;) {
+        any = false;
+        for (int i = 0; i < nb; ++i) {
+            if (!bs[i]->relax_active()) continue;
+            PO_HIP(hipSetDevice(bs[i]->ctx->device));
+            if ((rc = bs[i]->relax_issue(max_iter, record_interval))) return rc;
+        }
+        for (int i = 0; i < nb; ++i) {
+            if (!bs[i]->relax_active()) continue;
+            PO_HIP(hipSetDevice(bs[i]->ctx->device));
+            if ((rc = bs[i]->relax_collect(max_iter, min_iter, record_interval, thresh))) return rc;
+            any = any || bs[i]->relax_active();
+        }
+    }
+    int off = 0;
+    for (int i = 0; i < nb; ++i) {
+        bs[i]->relax_result(iters ? iters + off : nullptr, H ? H + off : nullptr, residual ? residual + off : nullptr);
+        off += bs[i]->size();
+    }
+    return PO_OK;
+}
+
 int po_batch_relax(po_batch *b, int max_iter, int min_iter, int record_interval, double thresh, int *iters,
                    double *H, double *residual) {
-    if (!b || max_iter < 1 || record_interval < 1) return fail(PO_ERR_INVALID, "bad argument");
-    PO_HIP(hipSetDevice(b->ctx->device));
-    return b->relax(max_iter, min_iter, record_interval, thresh, iters, H, residual);
+    return po_batches_relax(&b, 1, max_iter, min_iter, record_interval, thresh, iters, H, residual);
 }
 
 int po_batch_set_profiling(po_batch *b, int on) {

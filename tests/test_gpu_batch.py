@@ -218,3 +218,31 @@ def test_batch_relax_matches_models_relax(ctx64, monkeypatch):
     for a, m in zip(alone, batched):
         a.update(1)
         assert a.observe() == m.observe()
+
+
+def test_batches_relax_ragged_matches_models_relax(ctx64, monkeypatch):
+    """po_batches_relax: a ragged scan of two batches (16^3 and 32^3 grids,
+    different contour lengths), each on its own context, relaxed round-robin
+    with per-model stopping and record_interval 5, gives every model the
+    iterations, H and residual of po_models_relax on the same models alone."""
+    for k in PINS:
+        monkeypatch.setenv(k, "0")
+    specs = [((16, 16, 16), (9, 13), chiN) for chiN in (8.0, 12.0)] + \
+            [((32, 32, 32), (11, 21), chiN) for chiN in (7.0, 10.0, 13.0)]
+    alone, groups = [], {}
+    for i, (shape, Ms, chiN) in enumerate(specs):
+        wA = smooth_field(shape, 100 + i, amp=1.0, offset=chiN / 2)
+        wB = smooth_field(shape, 200 + i, amp=1.0, offset=chiN / 2)
+        kw = dict(cell=("CUBIC", 4.0), Ms=Ms, chiN=chiN, n_anderson=4)
+        alone.append(capi.ModelAB(ctx64, wA, wB, **kw))
+        if shape not in groups:
+            groups[shape] = (capi.Context(0, "fp64"), [])
+        c, ms = groups[shape]
+        ms.append(capi.ModelAB(c, wA, wB, **kw))
+    it0, H0, r0 = capi.relax_models(alone, 150, 10, 5, 1e-7)
+    batches = [capi.Batch(ms) for _, ms in groups.values()]
+    it1, H1, r1 = capi.relax_batches(batches, 150, 10, 5, 1e-7)
+    print(f"\n[batches relax] iters {it1.tolist()} residual {r1.tolist()}", flush=True)
+    assert np.array_equal(it0, it1) and np.array_equal(H0, H1) and np.array_equal(r0, r1)
+    for b in batches:
+        b.close()

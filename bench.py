@@ -311,7 +311,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--copy-probe", action="store_true", help="run only the traffic-calibration copy")
     ap.add_argument("--copy-rate", action="store_true", help="print the measured device copy rate only")
-    ap.add_argument("--form", choices=["all", "batch", "mixed", "fp64"], default="all",
+    ap.add_argument("--form", choices=["all", "batch", "mixed", "fp64", "string", "scan"], default="all",
                     help="time only one form (for profiling runs)")
     ap.add_argument("--systems", type=int, default=4, help="systems per batch of the headline form")
     ap.add_argument("--gpus", type=int, default=1)
@@ -332,6 +332,9 @@ def main():
         sys.exit(2)
     warmup = max(1, args.warmup)
     nsys = args.systems
+    if args.form in ("string", "scan"):  # one extras line alone (for profiling runs)
+        print(json.dumps(run_string() if args.form == "string" else run_scan()), flush=True)
+        return
 
     batch = mixed = f64 = first32 = first64 = None
     if args.form in ("all", "batch"):

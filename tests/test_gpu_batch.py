@@ -246,3 +246,30 @@ def test_batches_relax_ragged_matches_models_relax(ctx64, monkeypatch):
     assert np.array_equal(it0, it1) and np.array_equal(H0, H1) and np.array_equal(r0, r1)
     for b in batches:
         b.close()
+
+
+def test_batch_relax_with_adjoint(ctx32, monkeypatch):
+    """po_batch_relax with the fused adjoint (FP32): models leave the program
+    as they converge and the rebuilt program keeps their arenas; each model's
+    iterations, H and last fused stress equal those of the model relaxed
+    alone with its own fused adjoint."""
+    for k in PINS:
+        monkeypatch.setenv(k, "0")
+    monkeypatch.setenv("PO_SPLIT", "1")
+    alone, batched = [], []
+    for i, chiN in enumerate((8.0, 11.0, 13.0)):
+        wA = smooth_field((32, 32, 32), 300 + i, amp=1.0, offset=chiN / 2)
+        wB = smooth_field((32, 32, 32), 400 + i, amp=1.0, offset=chiN / 2)
+        kw = dict(cell=("ORTHORHOMBIC", 3.8, 4.0, 4.3), Ms=(13, 19), chiN=chiN, n_anderson=4)
+        a = capi.ModelAB(ctx32, wA, wB, **kw)
+        a.set_adjoint(True)
+        alone.append(a)
+        batched.append(capi.ModelAB(ctx32, wA, wB, **kw))
+    it0, H0, r0 = capi.relax_models(alone, 60, 5, 5, 1e-5)
+    b = capi.Batch(batched, adjoint=True)
+    it1, H1, r1 = b.relax(60, 5, 5, 1e-5)
+    print(f"\n[batch relax adjoint] iters {it1.tolist()}", flush=True)
+    assert np.array_equal(it0, it1) and np.array_equal(H0, H1) and np.array_equal(r0, r1)
+    for a, m in zip(alone, batched):
+        assert np.array_equal(a.last_stress(), m.last_stress())
+    b.close()

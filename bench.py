@@ -362,6 +362,8 @@ def main():
             "contour_steps_per_s": f64b["contour_steps_per_s"],
             "lockstep_step_us_per_system": f64b["lockstep_step_us_per_system"],
             "achieved_GBs": f64b["achieved_GBs"], "roofline_frac": f64b["achieved_GBs"] / HBM_PEAK_GBS}
+        if batch is not None:
+            extras["speedup_batched_vs_fp64_batched"] = batch["contour_steps_per_s"] / f64b["contour_steps_per_s"]
         print(f"[bench] fp64 batched {json.dumps(extras['fp64_plain_batched'])}", file=sys.stderr, flush=True)
         extras["string_batched"] = run_string()
         print(f"[bench] string {json.dumps(extras['string_batched'])}", file=sys.stderr, flush=True)

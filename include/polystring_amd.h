@@ -299,6 +299,11 @@ int po_batches_relax(po_batch **batches, int nb, int max_iter, int min_iter, int
 int po_batch_set_profiling(po_batch *b, int on);
 int po_batch_profile(po_batch *b, double *out4);
 int po_batch_launch_shape(po_batch *b, int *out5);
+/* Replay the shared propagator program from a hipGraph (on != 0): captured at
+ * the next update with every lane on the context's stream, and captured again
+ * after po_batch_relax rebuilds the program.  Same results as direct
+ * launches; no counterpart in the reference. */
+int po_batch_set_graph(po_batch *b, int on);
 
 #ifdef __cplusplus
 }

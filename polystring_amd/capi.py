@@ -48,7 +48,7 @@ EXPORTS = [
     "po_model_set_profiling", "po_model_profile", "po_model_launch_shape",
     "po_batch_create", "po_batch_destroy", "po_batch_update", "po_batch_contour_steps",
     "po_batch_set_profiling", "po_batch_profile", "po_batch_launch_shape", "po_string_set_sizes",
-    "po_batch_relax", "po_batches_relax",
+    "po_batch_relax", "po_batches_relax", "po_batch_set_graph",
 ]
 
 
@@ -124,6 +124,7 @@ def load():
         "po_string_set_sizes": ([_vp, _dp], i),
         "po_batch_relax": ([_vp, i, i, i, d, ctypes.POINTER(i), _dp, _dp], i),
         "po_batches_relax": ([ctypes.POINTER(_vp), i, i, i, i, d, ctypes.POINTER(i), _dp, _dp], i),
+        "po_batch_set_graph": ([_vp, i], i),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -395,6 +396,10 @@ class Batch:
 
     def set_profiling(self, on=True):
         _check(load().po_batch_set_profiling(self.h, 1 if on else 0))
+
+    def set_graph(self, on=True):
+        """Replay the shared program from a hipGraph (all lanes on one stream)."""
+        _check(load().po_batch_set_graph(self.h, 1 if on else 0))
 
     def profile(self):
         o = np.zeros(4)

@@ -934,6 +934,7 @@ struct po_batch {
     virtual int relax_collect(int max_iter, int min_iter, int record_interval, double thresh) = 0;
     virtual void relax_result(int *iters, double *H, double *residual) const = 0;
     virtual int size() const = 0;
+    virtual void set_graph(bool on) = 0;
     virtual void launch_shape(int *out5) = 0;
 };
 
@@ -1110,7 +1111,10 @@ template <typename T> struct Batch : po_batch {
         for (int it = 0; it < iters; ++it) {
             PO_HIP(mark());
             int rc;
-            if (dual) {
+            if (prog.graph_on()) {
+                // replay of the captured program: every lane on the context's stream
+                if ((rc = prog.run_graph(m0.sh, m0.rt, st))) return rc;
+            } else if (dual) {
                 PO_HIP(hipEventRecord(dev[0], st));
                 PO_HIP(hipStreamWaitEvent(st2, dev[0], 0));
                 rc = prog.run_dual(m0.sh, m0.rt, st, st2, dev[1], dev[2]);
@@ -1134,8 +1138,12 @@ template <typename T> struct Batch : po_batch {
         o[1] = prog.plane_variant;
         o[2] = prog.x_variant;
         o[3] = prog.hold;
-        o[4] = dual ? 1 : 0;
+        o[4] = (dual && !prog.graph_on()) ? 1 : 0;
     }
+
+    // hipGraph replay of the shared program (captured at the next update,
+    // again after each rebuild)
+    void set_graph(bool on) override { prog.use_graph = on ? 1 : 0; }
 
     ~Batch() override {
         if (registered)
@@ -1675,6 +1683,13 @@ int po_batch_profile(po_batch *b, double *out4) {
     out4[1] = rest;
     out4[2] = b->prof_iters;
     out4[3] = (double)b->lockstep;
+    return PO_OK;
+}
+
+int po_batch_set_graph(po_batch *b, int on) {
+    if (!b) return fail(PO_ERR_INVALID, "null batch");
+    PO_HIP(hipStreamSynchronize(b->ctx->stream));
+    b->set_graph(on != 0);
     return PO_OK;
 }
 

@@ -273,3 +273,34 @@ def test_batch_relax_with_adjoint(ctx32, monkeypatch):
     for a, m in zip(alone, batched):
         assert np.array_equal(a.last_stress(), m.last_stress())
     b.close()
+
+
+def test_batch_graph_replay_matches_direct(ctx32, monkeypatch):
+    """po_batch_set_graph: the shared program replayed from a hipGraph (every
+    lane on one stream) gives the same bits as direct one-stream launches,
+    over several replays with the models' own kernels and host reads in
+    between, and again after po_batch_relax rebuilds (and recaptures) it."""
+    for k in PINS:
+        monkeypatch.setenv(k, "0")
+    monkeypatch.setenv("PO_SPLIT", "1")
+    monkeypatch.setenv("PO_DUAL", "0")
+    runs = []
+    for graph in (False, True):
+        models = []
+        for s in SYSTEMS:
+            wA, wB, kw = make(ctx32, s, "RQM4")
+            models.append(capi.ModelAB(ctx32, wA, wB, **kw))
+        b = capi.Batch(models, adjoint=True)
+        if graph:
+            b.set_graph(True)
+        rec = []
+        for _ in range(3):
+            b.update(1)
+            rec.append([(m.observe(), m.last_stress().tolist()) for m in models])
+        it, H, r = b.relax(40, 5, 5, 1e-4)
+        rec.append((it.tolist(), H.tolist(), r.tolist()))
+        b.update(1)
+        rec.append([m.get("wA").tobytes() for m in models])
+        runs.append(rec)
+        b.close()
+    assert runs[0] == runs[1]

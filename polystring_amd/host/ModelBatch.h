@@ -20,6 +20,9 @@ public:
     // n Model_AB::update() of every model (src/Model_AB.cc:136-261), queued
     // without host synchronisation
     void update(int n = 1);
+    // scft::relax of every model (src/scft.cc:382-422,536-568), each stopping
+    // on its own; iters (optional) gets each model's iteration count
+    void relax(int max_iter, int min_iter, int record_interval, double thresh, std::vector<int> *iters = nullptr);
     std::size_t size() const { return _ms.size(); }
 
 private:

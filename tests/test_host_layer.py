@@ -158,6 +158,17 @@ int main() {
     a.update();  // the models stay usable once the batch is gone
     alone.update();
     std::printf("%.17g %.17g\n", a.H(), alone.H());
+    // batched relax against each model relaxed alone (po_model_relax)
+    Model_AB c("param.ini"), d("param.ini"), c1("param.ini");
+    std::vector<int> its;
+    {
+        ModelBatch batch({&c, &d});
+        batch.relax(300, 10, 1, 1e-9, &its);
+    }
+    int it1 = 0;
+    double h1 = 0, r1 = 0;
+    if (po_model_relax(c1.handle(), 300, 10, 1, 1e-9, &it1, &h1, &r1) != PO_OK) return 1;
+    std::printf("%d %d %d %.17g %.17g %.17g\n", its[0], its[1], it1, c.H(), d.H(), h1);
     return 0;
 }
 """
@@ -181,10 +192,12 @@ def test_cpp_model_batch_matches_models_alone(tmp_path):
     out = subprocess.run([str(exe)], cwd=tmp_path, capture_output=True, text=True, timeout=120, env=env)
     assert out.returncode == 0, out.stdout + out.stderr
     lines = [l.split() for l in out.stdout.strip().splitlines()]
-    assert len(lines) == 6
+    assert len(lines) == 7
     for l in lines[:5]:
         assert l[0] == l[1] == l[2], l
     assert lines[5][0] == lines[5][1]
+    r = lines[6]  # batched relax: iterations and H of both models equal those of the model relaxed alone
+    assert r[0] == r[1] == r[2] and r[3] == r[4] == r[5], r
 
 
 STRING_INI = """[IO]

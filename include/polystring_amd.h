@@ -259,8 +259,8 @@ int po_model_launch_shape(po_model *m, int *out5);
 
 /* ----------------------------------------------------------------- batch */
 /* Batched solves; no counterpart in the reference, which solves one system
- * per process.  n models of one context, one grid, one algorithm (RQM4 or
- * OS2) and one contour length share a single propagator launch program:
+ * per process.  n models of one context, one grid, one algorithm (RQM4,
+ * OS2 or ETDRK4) and one contour length share a single propagator launch program:
  * every launch advances the chains of all n systems (the forward chains on
  * the context's stream, the complementary chains on a second one).  Each
  * model keeps its own fields, Anderson history and observables, read with

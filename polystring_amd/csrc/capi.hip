@@ -960,7 +960,7 @@ template <typename T> struct Batch : po_batch {
         Model<T> &m0 = *ms[0];
         ctx = m0.ctx;
         adjoint = adj;
-        if (m0.desc.algo == PO_ETDRK4) return fail(PO_ERR_INVALID, "a batch runs RQM4 or OS2 programs");
+        if (m0.desc.algo == PO_ETDRK4 && adj) return fail(PO_ERR_INVALID, "ETDRK4 has no adjoint");
         for (auto *m : ms) {
             if (m->ctx != ctx) return fail(PO_ERR_INVALID, "batch models must share one context");
             if (m->sh.Lx != m0.sh.Lx || m->sh.Ly != m0.sh.Ly || m->sh.Lz != m0.sh.Lz ||

@@ -304,3 +304,27 @@ def test_batch_graph_replay_matches_direct(ctx32, monkeypatch):
         runs.append(rec)
         b.close()
     assert runs[0] == runs[1]
+
+
+def test_batch_etdrk4_matches_models_alone(ctx64, monkeypatch):
+    """ETDRK4 models (the Etdrk4_PBC updater, src/Etdrk4_PBC.cc) batched: the
+    same bits as updating each alone; the fused adjoint is refused (ETDRK4
+    has none)."""
+    for k in PINS:
+        monkeypatch.setenv(k, "0")
+    alone, batched = [], []
+    for s in SYSTEMS[:2]:
+        wA, wB, kw = make(ctx64, s, "ETDRK4")
+        alone.append(capi.ModelAB(ctx64, wA, wB, **kw))
+        batched.append(capi.ModelAB(ctx64, wA, wB, **kw))
+    with pytest.raises(capi.POError):
+        capi.Batch(batched, adjoint=True)
+    b = capi.Batch(batched)
+    for _ in range(3):
+        b.update()
+        for a, m in zip(alone, batched):
+            a.update()
+            assert a.observe() == m.observe()
+    for a, m in zip(alone, batched):
+        assert np.array_equal(a.get("phiA"), m.get("phiA"))
+    b.close()

@@ -199,8 +199,9 @@ unsigned Config::n_Anderson_mixing() const { return (unsigned)get_int("Algorithm
 
 AlgorithmCellType Config::algo_cell_optimization_type() const {
     const std::string s = upper(get_string("Algorithm_Cell_Optimization", "algorithm"));
-    if (s == "BRENT") return AlgorithmCellType::BRENT;
-    if (s == "MANUAL") return AlgorithmCellType::MANUAL;
+    // src/Config.cc:1334-1345
+    if (s == "BRENT" || s == "BRENT METHOD" || s == "NEWTON") return AlgorithmCellType::BRENT;
+    if (s == "MANUAL" || s == "BATCH") return AlgorithmCellType::MANUAL;
     return AlgorithmCellType::SINGLE;
 }
 

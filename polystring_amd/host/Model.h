@@ -26,6 +26,10 @@ public:
     virtual void save_field(const std::string file) = 0;
     virtual void save_density(const std::string file) = 0;
     virtual void save_q(const std::string file) = 0;
+    // New cell lengths with the current fields kept and the mixing history
+    // restarted (scft::reset_model -> Model::reset, src/scft.cc:356-380);
+    // false where the model cannot change its cell.
+    virtual bool set_cell(double a, double b, double c) { (void)a; (void)b; (void)c; return false; }
 
 protected:
     Config _cfg;

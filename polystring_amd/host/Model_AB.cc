@@ -199,3 +199,14 @@ void Model_AB::save_model(const std::string file) {
 }
 
 void Model_AB::save_q(const std::string file) { (void)file; /* slices stay on the GPU */ }
+
+bool Model_AB::set_cell(double a, double b, double c) {
+    _cfg.a(a);
+    _cfg.b(b);
+    _cfg.c(c);
+    _uc = UnitCell(_cfg);
+    const po_grid_desc g = _uc.grid_desc(_Lx, _Ly, _Lz);
+    po_host_check(po_model_set_cell(_m, g.a, g.b, g.c, g.alpha, g.beta, g.gamma), "po_model_set_cell");
+    _obs_valid = false;
+    return true;
+}

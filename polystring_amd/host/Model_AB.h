@@ -30,6 +30,7 @@ public:
     void save_field(const std::string file) override;
     void save_density(const std::string file) override;
     void save_q(const std::string file) override;
+    bool set_cell(double a, double b, double c) override;
 
     double Q() const;
     std::vector<double> field(int which) const;  // PO_FIELD_WA / WB / PHIA / PHIB

@@ -2,10 +2,13 @@
 #include "scft.h"
 
 #include <chrono>
+#include <cmath>
 #include <cstdlib>
 #include <iostream>
 #include <sstream>
 
+#include "Model_AB.h"
+#include "Propagator.h"
 #include "UnitCell.h"
 #include "mat5.h"
 
@@ -17,11 +20,95 @@ scft::scft(const std::string config_file, Model *pmodel) : _cfg(config_file.c_st
 }
 
 void scft::run() {
-    if (_cfg.algo_cell_optimization_type() != AlgorithmCellType::SINGLE) {
-        std::cerr << "Cell optimization algorithm other than Single is not available in this build." << std::endl;
-        std::exit(1);
+    const AlgorithmCellType t = _cfg.algo_cell_optimization_type();
+    if (t == AlgorithmCellType::MANUAL) {
+        // src/scft.cc:82-95: every size from the previous size's fields
+        // (reset_model keeps them), relaxed and saved by save_data.  A
+        // non-positive step or a missing bound gives that length one value.
+        const std::string sec = "Algorithm_Cell_Optimization";
+        std::vector<double> lo = _cfg.get_list(sec, "batch_cell_min"), st = _cfg.get_list(sec, "batch_cell_step"),
+                            hi = _cfg.get_list(sec, "batch_cell_max");
+        lo.resize(3, 0.0);
+        st.resize(3, 0.0);
+        if (hi.size() < 3) hi.resize(3, -1e300);
+        for (int d = 0; d < 3; ++d)
+            if (!(st[d] > 0) || hi[d] < lo[d]) hi[d] = lo[d], st[d] = 1.0;
+        for (double a = lo[0]; a <= hi[0]; a += st[0])
+            for (double b = lo[1]; b <= hi[1]; b += st[1])
+                for (double c = lo[2]; c <= hi[2]; c += st[2]) {
+                    reset_model(a, b, c);
+                    _pmodel->display_parameters();
+                    relax();
+                }
+        return;
+    }
+    if (t == AlgorithmCellType::BRENT) {
+        // The reference brackets the optimum and runs Brent's search on H(a)
+        // (src/scft.cc:42-75,124-309).  This build replaces the search with
+        // the stress-driven relaxation: fields relaxed, dH/dl from the adjoint,
+        // secant steps on the free lengths until |dH/dl| < tol_cell.
+        auto *m = dynamic_cast<Model_AB *>(_pmodel);
+        if (!m) {
+            std::cerr << "Brent cell optimization needs a Model_AB" << std::endl;
+            std::exit(1);
+        }
+        const std::string sec = "Algorithm_Cell_Optimization";
+        const double tol = _cfg.get_double(sec, "tol_cell", 1e-5);
+        const int max_outer = _cfg.get_int(sec, "max_iter_cell", 30);
+        double cell6[6], h = 0.0, stress[3];
+        int outer = 0;
+        po_host_check(po_model_relax_cell(m->handle(), max_outer, tol, (int)_cfg.max_iter(), (int)_cfg.min_iter(),
+                                          (int)_cfg.record_interval(), _cfg.thresh_residual(), cell6, &h, stress,
+                                          &outer),
+                      "po_model_relax_cell");
+        m->invalidate();
+        _cfg.a(cell6[0]);
+        _cfg.b(cell6[1]);
+        _cfg.c(cell6[2]);
+        UnitCell uc(_cfg);
+        _curr_a = uc.lx();
+        _curr_b = uc.ly();
+        _curr_c = uc.lz();
+        _as.push_back(_curr_a);
+        _bs.push_back(_curr_b);
+        _cs.push_back(_curr_c);
+        _Fs.push_back(h);
+        _H.push_back(h);
+        _residual_error.push_back(m->residual_error());
+        const std::string s = _cfg.base_dir() + (_cfg.data_file().empty() ? std::string("scft_out") : _cfg.data_file());
+        save_cell(s + "_cell.mat");
+        std::cout << "amin = " << _curr_a << " bmin = " << _curr_b << " cmin = " << _curr_c << std::endl;
+        std::cout << "Fmin = " << h << " (stress-driven relaxation, " << outer << " outer steps, |dH/dl| "
+                  << std::fabs(stress[0]) << ")" << std::endl;
+        return;
     }
     solve_scft(_cfg.a(), _cfg.b(), _cfg.c());
+}
+
+// scft::reset_model (src/scft.cc:356-380): new lengths, the record labels
+// from UnitCell's lx, ly, lz, and the model continued from its fields.
+void scft::reset_model(double a, double b, double c) {
+    _cfg.a(a);
+    _cfg.b(b);
+    _cfg.c(c);
+    UnitCell uc(_cfg);
+    _curr_a = uc.lx();
+    _curr_b = uc.ly();
+    _curr_c = uc.lz();
+    if (!_pmodel->set_cell(a, b, c)) {
+        std::cerr << "the model cannot change its cell" << std::endl;
+        std::exit(1);
+    }
+}
+
+void scft::save_cell(const std::string file) const {
+    mat5::Writer w(file);
+    if (!w.ok()) return;
+    const int n = (int)_Fs.size();
+    w.put("a", {n, 1}, _as.data());
+    w.put("b", {n, 1}, _bs.data());
+    w.put("c", {n, 1}, _cs.data());
+    w.put("F", {n, 1}, _Fs.data());
 }
 
 double scft::solve_scft(double a, double b, double c) {

@@ -283,3 +283,48 @@ def test_string_driver_matches_library(tmp_path, monkeypatch):
     phia = r["phia"].reshape((m,) + shape)
     assert np.array_equal(phia[0], ends[0][2]) and np.array_equal(phia[-1], ends[1][2])
     assert np.array_equal(phia[1:-1], s.get("phiA").reshape((m,) + shape)[1:-1])
+
+
+@pytest.mark.gpu
+def test_driver_cell_optimization(tmp_path):
+    """[Algorithm_Cell_Optimization] through polyorder_amd on the 1-D LAM case:
+    Brent runs the stress-driven relaxation that replaces the reference's
+    Brent search and lands on the recorded optimum a* = 4.476742,
+    F = 4.7456258048 (example/AB/LAM/1D/cell_opt_rqm4_anderson.mat), writing
+    <data_file>_cell.mat (src/scft.cc:589-609); Batch relaxes the
+    batch_cell_min/step/max sizes in turn (src/scft.cc:82-95), each from the
+    previous size's fields, saving each size's data."""
+    from polystring_amd import capi
+    f = np.load(os.path.join(GOLD, "lam1d_field.npz"))
+    sio.savemat(str(tmp_path / "field_in_64.mat"), {"wA": f["wA"], "wB": f["wB"]})
+    brent = INI.replace("algorithm = Single", "algorithm = Brent\ntol_cell = 1.0e-7\nmax_iter_cell = 30")
+    (tmp_path / "param.ini").write_text(brent)
+    print("\n[driver] Brent", flush=True)
+    out = subprocess.run([os.path.join(PKG, "polyorder_amd"), "param.ini"], cwd=tmp_path, capture_output=True,
+                         text=True, timeout=200)
+    assert out.returncode == 0, out.stdout + out.stderr
+    cell = sio.loadmat(str(tmp_path / "scft_out_cell.mat"))
+    a, F = float(cell["a"].ravel()[-1]), float(cell["F"].ravel()[-1])
+    print(f"[driver] Brent replacement: a {a!r} F {F!r}", flush=True)
+    assert abs(a - 4.476742) < 3e-4
+    assert 4.7456258048 - 1e-8 <= F <= 4.7456258048 + 1e-9
+    # batch scan over a = 4.0, 4.2, 4.4
+    batch = INI.replace("algorithm = Single", "algorithm = Batch\nbatch_cell_min = [4.0, 0, 0]\n"
+                        "batch_cell_step = [0.2, 0, 0]\nbatch_cell_max = [4.45, 0, 0]")
+    d2 = tmp_path / "batch"
+    d2.mkdir()
+    sio.savemat(str(d2 / "field_in_64.mat"), {"wA": f["wA"], "wB": f["wB"]})
+    (d2 / "param.ini").write_text(batch)
+    print("[driver] Batch", flush=True)
+    out = subprocess.run([os.path.join(PKG, "polyorder_amd"), "param.ini"], cwd=d2, capture_output=True, text=True,
+                         timeout=200)
+    assert out.returncode == 0, out.stdout + out.stderr
+    saved = sorted(p for p in os.listdir(d2) if p.startswith("scft_out_a") and p.endswith(".mat"))
+    labels = sorted({p.split("_b")[0] for p in saved})
+    assert labels == ["scft_out_a4", "scft_out_a4.2", "scft_out_a4.4"], saved
+    done = dict(l.split(":\t", 1) for l in (d2 / "done.txt").read_text().splitlines() if ":\t" in l)
+    ctx = capi.Context(0, "fp64")
+    m = capi.ModelAB(ctx, f["wA"], f["wB"], cell=("LAMELLAR", 4.4), Ms=(51, 51), chiN=30.0, lam=(0.05, 0.05),
+                     n_anderson=5, dim=1)
+    _, H44, _ = m.relax(2000, 10, 1, 1e-10)
+    assert abs(float(done["H"]) - H44) <= 1e-9 * H44

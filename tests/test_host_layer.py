@@ -328,3 +328,38 @@ def test_driver_cell_optimization(tmp_path):
                      n_anderson=5, dim=1)
     _, H44, _ = m.relax(2000, 10, 1, 1e-10)
     assert abs(float(done["H"]) - H44) <= 1e-9 * H44
+
+
+@pytest.mark.gpu
+def test_string_driver_reads_saved_string(tmp_path, monkeypatch):
+    """polystring_AB_amd with is_readString: the string starts from
+    stringData.mat's 4-D wa, wb (m x Lx x Ly x Lz, Fortran order,
+    scft/polystring_AB.cc:186-190) and matches the String binding started
+    from the same images, bit for bit (launch shapes pinned)."""
+    from polystring_amd import capi
+    pins = dict(PO_SPLIT="0", PO_PLANE_VARIANT="0", PO_XPASS_VARIANT="0", PO_PLANE_HOLD="0")
+    for k, v in pins.items():
+        monkeypatch.setenv(k, v)
+    m, shape = 5, (32, 32)
+    rng = np.random.default_rng(9)
+    wa = 7.5 + 0.6 * rng.standard_normal((m,) + shape)   # distinct images, asymmetric in x and y
+    wb = 15.0 - wa + 0.1 * rng.standard_normal((m,) + shape)
+    phia = 0.5 + 0.05 * rng.standard_normal((m,) + shape)
+    sio.savemat(str(tmp_path / "stringData.mat"), {"wa": wa[..., None], "wb": wb[..., None], "phia": phia[..., None],
+                                                   "phib": 1.0 - phia[..., None]})
+    ini = STRING_INI.replace("is_readString = false", "is_readString = true").replace(
+        "is_changeSize = true", "is_changeSize = false").replace("string_iteration = 3", "string_iteration = 2")
+    (tmp_path / "paramString.ini").write_text(ini)
+    out = subprocess.run([os.path.join(PKG, "polystring_AB_amd")], cwd=tmp_path, capture_output=True, text=True,
+                         timeout=120, env=dict(os.environ, **pins))
+    assert out.returncode == 0, out.stdout + out.stderr
+    r = sio.loadmat(str(tmp_path / "result.mat"))
+    ctx = capi.Context(0, "fp64")
+    s = capi.String(ctx, wa[..., None], wb[..., None], cell=("RECTANGULAR", 4.0, 4.4), Ms=(11, 11), chiN=15.0,
+                    lam=(0.015, 0.015, 100.0), compressible=True, mirror=True, dim=2)
+    s.set_sizes(np.full(m, 4.0))  # without is_changeSize every image takes a
+    H = [s.iterate(3, 1, 1, 1e-10)[0] for _ in range(2)]
+    assert np.array_equal(r["H"], np.array(H))
+    assert np.array_equal(r["wa"].reshape((m,) + shape), s.get("wA").reshape((m,) + shape))
+    phia_out = r["phia"].reshape((m,) + shape)
+    assert np.array_equal(phia_out[0], phia[0]) and np.array_equal(phia_out[-1], phia[-1])

@@ -22,8 +22,9 @@ int main(int argc, char **argv) {
     Config cfg(argv[1]);
     printf("%u %u %u %u\n", cfg.dim(), cfg.Lx(), cfg.Ly(), cfg.Lz());
     printf("%.17g %.17g %.17g\n", cfg.a(), cfg.chiN().at(0), cfg.thresh_residual());
-    printf("%u %u %u %d %d %d\n", cfg.Ms()[0], cfg.Ms()[1], cfg.n_Anderson_mixing(),
-           (int)cfg.algo_mde_type(), (int)cfg.get_crystal_system_type(), cfg.etdrk4_M());
+    printf("%u %u %u %d %d %d %d\n", cfg.Ms()[0], cfg.Ms()[1], cfg.n_Anderson_mixing(),
+           (int)cfg.algo_mde_type(), (int)cfg.get_crystal_system_type(), cfg.etdrk4_M(),
+           (int)cfg.algo_cell_optimization_type());
     std::map<std::string, mat5::Var> v;
     std::string err;
     if (!mat5::read(argv[2], v, err)) { printf("ERR %s\n", err.c_str()); return 1; }
@@ -90,7 +91,7 @@ def test_config_and_mat5_roundtrip(tmp_path):
     lines = out.split("\n")
     assert lines[0].split() == ["1", "64", "1", "1"]
     assert [float(x) for x in lines[1].split()] == [4.0, 30.0, 1e-10]
-    assert lines[2].split() == ["51", "51", "5", "2", "0", "0"]   # RQM4, LAMELLAR, etdrk4_M absent
+    assert lines[2].split() == ["51", "51", "5", "2", "0", "0", "0"]   # RQM4, LAMELLAR, etdrk4_M absent, Single
     assert [float(x) for x in lines[3].split()] == [w[0, 0, 0], w[1, 2, 1]]
     back = sio.loadmat(str(tmp_path / "out.mat"))
     assert np.array_equal(back["wA"], w)
@@ -100,7 +101,14 @@ def test_config_and_mat5_roundtrip(tmp_path):
         ini.write_text(INI.replace("algorithm = RQM4", f"algorithm = {word}\netdrk4_M = 32"))
         out = subprocess.check_output([str(exe), str(ini), str(tmp_path / "in.mat"), str(tmp_path / "o2.mat")],
                                       text=True)
-        assert out.split("\n")[2].split()[3:] == [code, "0", "32"], word
+        assert out.split("\n")[2].split()[3:6] == [code, "0", "32"], word
+    # [Algorithm_Cell_Optimization] words (src/Config.cc:1334-1345): 0 SINGLE, 1 BRENT, 2 MANUAL
+    for word, code in (("Single", "0"), ("none", "0"), ("Brent", "1"), ("Newton", "1"), ("Manual", "2"),
+                       ("Batch", "2"), ("bogus", "0")):
+        ini.write_text(INI.replace("algorithm = Single", f"algorithm = {word}"))
+        out = subprocess.check_output([str(exe), str(ini), str(tmp_path / "in.mat"), str(tmp_path / "o3.mat")],
+                                      text=True)
+        assert out.split("\n")[2].split()[6] == code, word
 
 
 @pytest.mark.gpu

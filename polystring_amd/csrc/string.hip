@@ -57,6 +57,7 @@ struct EmArgs {
     int compressible;
     double cA2, cA4, cB2, cB4;  // -ds/2, -ds/4 per block
     double *part;               // [img][blk][EM_NV]
+    int etd;                    // ETDRK4: the factor tables hold -w (src/Etdrk4_PBC.cc:136)
 };
 
 // Simple mixing of every active image (src/Model_AB.cc:205-229,254-258;
@@ -88,10 +89,15 @@ __global__ void __launch_bounds__(kStrThreads) k_str_em(EmArgs a, T *eA2, T *eA4
         const double wb = a.wB[j] * (1.0 - a.lamB) + a.lamB * tb;
         a.wA[j] = wa;
         a.wB[j] = wb;
-        eA2[j] = (T)exp(a.cA2 * wa);
-        eA4[j] = (T)exp(a.cA4 * wa);
-        eB2[j] = (T)exp(a.cB2 * wb);
-        eB4[j] = (T)exp(a.cB4 * wb);
+        if (a.etd) {
+            eA2[j] = (T)(-1.0 * wa);
+            eB2[j] = (T)(-1.0 * wb);
+        } else {
+            eA2[j] = (T)exp(a.cA2 * wa);
+            eA4[j] = (T)exp(a.cA4 * wa);
+            eB2[j] = (T)exp(a.cB2 * wb);
+            eB4[j] = (T)exp(a.cB4 * wb);
+        }
         double hw = a.chiN * pa * pb - wa * pa - wb * pb;
         if (a.compressible) hw += a.lamY / 2.0 * inc * inc;
         acc[EM_HW] += hw;
@@ -254,6 +260,8 @@ template <typename T> struct StringAB : po_string {
     DBuf<T> eA2, eA4, eB2, eB4, qA, qB, qAc, qBc;
     DBuf<cx<T>> spec;
     KTable<T> kA, kAh, kB, kBh;
+    EtdTables<T> etdA, etdB;
+    std::unique_ptr<DBuf<cx<T>>[]> etdState;  // ETDRK4: v, acc, t of each of the 2m lanes
     // per-image cells (set_sizes): image i's tables kimg[4i .. 4i+3] = A, A half, B, B half
     std::vector<std::unique_ptr<KTable<T>>> kimg;
     Program<T> prog;
@@ -268,7 +276,8 @@ template <typename T> struct StringAB : po_string {
         if (d.m < 3 || d.m > kStrMaxImages)
             return fail(PO_ERR_INVALID, "a string needs 3 to " + std::to_string(kStrMaxImages) + " images");
         if (d.sA < 2 || d.sB < 2) return fail(PO_ERR_INVALID, "Ms entries must be >= 2");
-        if (d.algo != PO_RQM4 && d.algo != PO_OS2) return fail(PO_ERR_INVALID, "algo must be RQM4 or OS2");
+        if (d.algo != PO_RQM4 && d.algo != PO_OS2 && d.algo != PO_ETDRK4)
+            return fail(PO_ERR_INVALID, "algo must be RQM4, OS2 or ETDRK4");
         m = d.m;
         sA = d.sA;
         sB = d.sB;
@@ -326,6 +335,14 @@ template <typename T> struct StringAB : po_string {
             if ((rc = build_ktable<T>(kAh, sh, cell, false, 0.5 * dsA, st))) return rc;
             if ((rc = build_ktable<T>(kBh, sh, cell, false, 0.5 * dsB, st))) return rc;
         }
+        if (d.algo == PO_ETDRK4) {
+            // Etdrk4_PBC coefficients (one set when both blocks share ds) and
+            // every lane's carried spectra v, acc, t
+            if ((rc = build_etd_tables<T>(etdA, sh, cell, dsA, st))) return rc;
+            if (dsA != dsB && (rc = build_etd_tables<T>(etdB, sh, cell, dsB, st))) return rc;
+            etdState.reset(new DBuf<cx<T>>[6 * (size_t)m]);
+            for (int i = 0; i < 6 * m; ++i) PO_HIP(etdState[i].alloc(etd_slots<T>(sh)));
+        }
         for (int i = 0; i < m; ++i) {
             PO_HIP(launch_fill<T>(qA.p + (size_t)i * sA * M, (T)1, M, st));   // heads qA[0] = qBc[0] = 1
             PO_HIP(launch_fill<T>(qBc.p + (size_t)i * sB * M, (T)1, M, st));  // (src/Model_AB.cc:779-782)
@@ -357,8 +374,15 @@ template <typename T> struct StringAB : po_string {
             lanes[2 * i + 1].segs = {Segment<T>{bc, sB, eB2.p + o, eB4.p + o, kb, kbh},
                                      Segment<T>{ac, sA, eA2.p + o, eA4.p + o, ka, kah}};
             for (int l = 0; l < 2; ++l) {
-                lanes[2 * i + l].Sf = spec.p + (size_t)(4 * i + 2 * l) * sh.Mh;
-                lanes[2 * i + l].Sh = spec.p + (size_t)(4 * i + 2 * l + 1) * sh.Mh;
+                Lane<T> &ln = lanes[2 * i + l];
+                ln.Sf = spec.p + (size_t)(4 * i + 2 * l) * sh.Mh;
+                ln.Sh = spec.p + (size_t)(4 * i + 2 * l + 1) * sh.Mh;
+                if (desc.algo == PO_ETDRK4) {
+                    for (auto &sg : ln.segs) sg.etd = (sg.q == a || sg.q == ac || dsA == dsB) ? &etdA : &etdB;
+                    ln.ev = etdState[6 * i + 3 * l].p;
+                    ln.eacc = etdState[6 * i + 3 * l + 1].p;
+                    ln.et = etdState[6 * i + 3 * l + 2].p;
+                }
             }
         }
         prog.reset();
@@ -370,6 +394,7 @@ template <typename T> struct StringAB : po_string {
     // (1-D and 2-D strings) or b (3-D); the crystal system ties the other
     // lengths as UnitCell does.  The launch shape stays the tuned one.
     int set_sizes(const double *size) override {
+        if (desc.algo == PO_ETDRK4) return fail(PO_ERR_INVALID, "per-image cell sizes are not built for ETDRK4");
         hipStream_t st = ctx->stream;
         PO_HIP(hipStreamSynchronize(st));
         std::vector<std::unique_ptr<KTable<T>>> k(4 * (size_t)m);
@@ -400,6 +425,11 @@ template <typename T> struct StringAB : po_string {
     int factors() {
         hipStream_t st = ctx->stream;
         const long n = (long)m * sh.M;
+        if (desc.algo == PO_ETDRK4) {  // -w, the nonlinear term of Etdrk4_PBC (src/Etdrk4_PBC.cc:136)
+            PO_HIP(launch_scale_field<T>(wA.p, eA2.p, -1.0, n, st));
+            PO_HIP(launch_scale_field<T>(wB.p, eB2.p, -1.0, n, st));
+            return PO_OK;
+        }
         PO_HIP(launch_exp_factor<T>(wA.p, eA2.p, -0.5 * dsA, n, st));
         PO_HIP(launch_exp_factor<T>(wA.p, eA4.p, -0.25 * dsA, n, st));
         PO_HIP(launch_exp_factor<T>(wB.p, eB2.p, -0.5 * dsB, n, st));
@@ -451,6 +481,7 @@ template <typename T> struct StringAB : po_string {
         a.cB2 = -0.5 * dsB;
         a.cB4 = -0.25 * dsB;
         a.part = part.p;
+        a.etd = desc.algo == PO_ETDRK4 ? 1 : 0;
         hipLaunchKernelGGL(k_str_em<T>, dim3(kStrBlocks, m), dim3(kStrThreads), 0, st, a, eA2.p, eA4.p, eB2.p,
                            eB4.p);
         PO_HIP(hipGetLastError());

@@ -154,9 +154,11 @@ int main(int argc, char *argv[]) {
     d.lamA = lam.size() > 0 ? lam[0] : 0.1;
     d.lamB = lam.size() > 1 ? lam[1] : 0.1;
     d.lamY = lam.size() > 2 ? lam[2] : 1.0;
-    d.algo = cfg.algo_mde_type() == AlgorithmMDEType::OS2 ? PO_OS2 : PO_RQM4;
-    if (cfg.algo_mde_type() == AlgorithmMDEType::ETDRK4) {
-        std::cerr << "polystring: the string method runs RQM4 or OS2" << std::endl;
+    d.algo = cfg.algo_mde_type() == AlgorithmMDEType::OS2    ? PO_OS2
+             : cfg.algo_mde_type() == AlgorithmMDEType::RQM4 ? PO_RQM4
+                                                              : PO_ETDRK4;
+    if (d.algo == PO_ETDRK4 && is_changeSize) {
+        std::cerr << "polystring: per-image cell sizes (is_changeSize) are not built for ETDRK4" << std::endl;
         return 1;
     }
     d.contour = cfg.algo_contour_integration_type() == AlgorithmContourType::TRAPEZOIDAL ? PO_QUAD_TRAP : PO_QUAD_AUTO;
@@ -165,7 +167,8 @@ int main(int argc, char *argv[]) {
     d.m = m;
     po_string *str = nullptr;
     check(po_string_create(po_host_context(&cfg), &d, wa.data(), wb.data(), &str), "po_string_create");
-    check(po_string_set_sizes(str, stringSize.data()), "po_string_set_sizes");
+    // ETDRK4 keeps the configured cell for every image (no per-image tables)
+    if (d.algo != PO_ETDRK4) check(po_string_set_sizes(str, stringSize.data()), "po_string_set_sizes");
 
     // scft::run of each image: [Algorithm_SCFT] min_iter, max_iter, thresh_residual; [IO] record_interval
     const int max_iter = (int)cfg.max_iter(), min_iter = (int)cfg.min_iter();

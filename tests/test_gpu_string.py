@@ -167,3 +167,30 @@ def test_reference_string_grid_reproduces_endpoint_record(ctx64):
     H, S = s.iterate(1, 1, 1, 0.0)
     print(f"\n[ref string grid] H {H.tolist()} recorded {float(g['F'])!r}", flush=True)
     assert np.abs(H - float(g["F"])).max() <= 1e-11 * float(g["F"])
+
+
+@pytest.mark.parametrize("shape,cell,dim,Ms,compressible,mirror,lam", [CASES[2], CASES[1]])
+def test_string_etdrk4_matches_oracle(ctx64, shape, cell, dim, Ms, compressible, mirror, lam):
+    """The string with the ETDRK4 updater (Etdrk4_PBC, src/Etdrk4_PBC.cc): two
+    string iterations against the oracle's string with ETDRK4 images, 1e-10
+    on H, S, wA and phiA."""
+    m, chiN = 5, 15.0
+    print(f"\n[string etdrk4] {shape} mirror={mirror}", flush=True)
+    wa, wb = initial_string(shape, m, chiN, 0.5)
+    relax = (3, 1, 1, 1e-10)
+    models = [O.EMModel(wa[s], wb[s], dim=dim, cell=cell, Ms=Ms, chiN=chiN, lam=lam, mirror=mirror,
+                        compressible=compressible, mde="ETDRK4") for s in range(m)]
+    ends = (wa[0].copy(), wb[0].copy(), wa[-1].copy(), wb[-1].copy())
+    g = capi.String(ctx64, wa, wb, cell=cell, Ms=Ms, chiN=chiN, lam=lam, compressible=compressible, mirror=mirror,
+                    dim=dim, algo="ETDRK4")
+    owa, owb = wa, wb
+    for it in range(2):
+        H, S = g.iterate(*relax)
+        Ho, So, owa, owb, opa, opb = O.string_iterate(models, owa, owb, ends, relax)
+        print(f"  it {it}: H {H} oracle {Ho}", flush=True)
+        assert relerr(H, Ho) <= 1e-10
+        assert np.abs(S - So).max() <= 1e-10
+        assert relerr(g.get("wA"), owa) <= 1e-10
+        assert relerr(g.get("phiA"), opa) <= 1e-10
+    with pytest.raises(capi.POError):
+        g.set_sizes(np.full(m, 4.0))

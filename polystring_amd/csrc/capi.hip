@@ -928,6 +928,7 @@ struct po_batch {
         return hipEventRecord(ev[ev_used++], ctx->stream);
     }
     virtual int update(int n) = 0;
+    virtual int restore_all() = 0;
     virtual int relax_begin() = 0;
     virtual bool relax_active() const = 0;
     virtual int relax_issue(int max_iter, int record_interval) = 0;
@@ -1047,6 +1048,15 @@ template <typename T> struct Batch : po_batch {
     }
 
     bool relax_active() const override { return r_left > 0; }
+
+    // A relax that returned an error part-way leaves the program built over
+    // the models still active; a plain update runs every model again.
+    int restore_all() override {
+        if (ms.size() == all.size()) return PO_OK;
+        r_left = 0;
+        PO_HIP(hipStreamSynchronize(ctx->stream));
+        return build_program(all);
+    }
 
     int relax_issue(int max_iter, int record_interval) override {
         if (r_left == 0) return PO_OK;
@@ -1612,7 +1622,8 @@ int po_batch_destroy(po_batch *b) {
 int po_batch_update(po_batch *b, int n_iter) {
     if (!b || n_iter < 0) return fail(PO_ERR_INVALID, "bad argument");
     PO_HIP(hipSetDevice(b->ctx->device));
-    return b->update(n_iter);
+    int rc = b->restore_all();
+    return rc ? rc : b->update(n_iter);
 }
 
 long po_batch_contour_steps(const po_batch *b) { return b ? b->steps : -1; }

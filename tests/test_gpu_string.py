@@ -169,7 +169,7 @@ def test_reference_string_grid_reproduces_endpoint_record(ctx64):
     assert np.abs(H - float(g["F"])).max() <= 1e-11 * float(g["F"])
 
 
-@pytest.mark.parametrize("shape,cell,dim,Ms,compressible,mirror,lam", [CASES[2], CASES[1]])
+@pytest.mark.parametrize("shape,cell,dim,Ms,compressible,mirror,lam", [CASES[2], CASES[1], CASES[0]])
 def test_string_etdrk4_matches_oracle(ctx64, shape, cell, dim, Ms, compressible, mirror, lam):
     """The string with the ETDRK4 updater (Etdrk4_PBC, src/Etdrk4_PBC.cc): two
     string iterations against the oracle's string with ETDRK4 images, 1e-10

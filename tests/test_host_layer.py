@@ -250,12 +250,14 @@ algorithm = Simpson
 
 
 @pytest.mark.gpu
-def test_string_driver_matches_library(tmp_path, monkeypatch):
+@pytest.mark.parametrize("algo", ["RQM4", "ETDRK4"])
+def test_string_driver_matches_library(tmp_path, monkeypatch, algo):
     """polystring_AB_amd (the scft/polystring_AB.cc flow): paramString.ini,
-    defect.mat / vLam.mat ends, is_changeSize, 3 string iterations, result.mat
-    every iteration; checked bit for bit against the String binding on the
-    same inputs (launch shapes pinned in both processes: each autotunes, and
-    variants agree only to rounding)."""
+    defect.mat / vLam.mat ends, is_changeSize (RQM4), 3 string iterations,
+    result.mat every iteration; checked bit for bit against the String binding
+    on the same inputs (launch shapes pinned in both processes: each
+    autotunes, and variants agree only to rounding).  ETDRK4 images keep one
+    cell, so the driver refuses ETDRK4 with is_changeSize."""
     from polystring_amd import capi
     pins = dict(PO_SPLIT="0", PO_PLANE_VARIANT="0", PO_XPASS_VARIANT="0", PO_PLANE_HOLD="0")
     for k, v in pins.items():
@@ -269,8 +271,16 @@ def test_string_driver_matches_library(tmp_path, monkeypatch):
         phiA = 0.5 + 0.1 * rng.standard_normal(shape)
         sio.savemat(str(tmp_path / name), {"wA": wA, "wB": wB, "phiA": phiA, "phiB": 1.0 - phiA})
         ends.append((wA, wB, phiA))
-    (tmp_path / "paramString.ini").write_text(STRING_INI)
-    print("\n[host] running polystring_AB_amd", flush=True)
+    ini = STRING_INI
+    if algo == "ETDRK4":
+        ini = ini.replace("algorithm = RQM4", "algorithm = ETDRK4\netdrk4_M = 32")
+        (tmp_path / "paramString.ini").write_text(ini)
+        out = subprocess.run([os.path.join(PKG, "polystring_AB_amd")], cwd=tmp_path, capture_output=True,
+                             text=True, timeout=120, env=dict(os.environ, **pins))
+        assert out.returncode != 0 and "is_changeSize" in out.stderr, out.stdout + out.stderr
+        ini = ini.replace("is_changeSize = true", "is_changeSize = false")
+    (tmp_path / "paramString.ini").write_text(ini)
+    print(f"\n[host] running polystring_AB_amd ({algo})", flush=True)
     out = subprocess.run([os.path.join(PKG, "polystring_AB_amd")], cwd=tmp_path, capture_output=True, text=True,
                          timeout=120, env=dict(os.environ, **pins))
     assert out.returncode == 0, out.stdout + out.stderr
@@ -281,8 +291,9 @@ def test_string_driver_matches_library(tmp_path, monkeypatch):
     wb0 = ends[0][1] + t * (ends[1][1] - ends[0][1])
     ctx = capi.Context(0, "fp64")
     s = capi.String(ctx, wa0[..., None], wb0[..., None], cell=("RECTANGULAR", 4.0, 4.4), Ms=(11, 11), chiN=15.0,
-                    lam=(0.015, 0.015, 100.0), compressible=True, mirror=True, dim=2)
-    s.set_sizes(3.8 + np.arange(m) / (m - 1) * (4.4 - 3.8))  # size_S + s/(m-1) (size_E - size_S)
+                    lam=(0.015, 0.015, 100.0), compressible=True, mirror=True, dim=2, algo=algo)
+    if algo == "RQM4":
+        s.set_sizes(3.8 + np.arange(m) / (m - 1) * (4.4 - 3.8))  # size_S + s/(m-1) (size_E - size_S)
     H = [s.iterate(3, 1, 1, 1e-10)[0] for _ in range(3)]
     assert np.array_equal(r["H"], np.array(H))
     assert np.allclose(r["F"].ravel(), np.mean(H, axis=1), rtol=0, atol=1e-15)

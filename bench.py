@@ -159,6 +159,15 @@ def run_batch(precision, nsys, steps, warmup, adjoint, L=128, Ns=200):
     return out
 
 
+def run_etdrk4(steps=10, warmup=2):
+    """The 128^3 Ns=200 system with the ETDRK4 updater, FP64, no adjoint."""
+    etd, _ = run_form("fp64", steps, warmup, adjoint=False, algo="ETDRK4")
+    return {"workload": "the same 128^3 Ns=200 system with the ETDRK4 updater (Etdrk4_PBC), FP64",
+            "contour_steps_per_s": etd["contour_steps_per_s"], "iters_per_s": etd["iters_per_s"],
+            "lockstep_step_us": etd["lockstep_step_us"], "achieved_GBs": etd["achieved_GBs"],
+            "bytes_per_launch": etd["bytes_per_launch"], "H": etd["H"]}
+
+
 def run_string(iters=2, precision="fp32"):
     """The batched string method on the 32-image 64^3 string (f=0.33,
     chiN=17.5, Ns=150): per string iteration every image takes 10 EM updates,
@@ -311,7 +320,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--copy-probe", action="store_true", help="run only the traffic-calibration copy")
     ap.add_argument("--copy-rate", action="store_true", help="print the measured device copy rate only")
-    ap.add_argument("--form", choices=["all", "batch", "mixed", "fp64", "string", "scan"], default="all",
+    ap.add_argument("--form", choices=["all", "batch", "mixed", "fp64", "string", "scan", "etdrk4"], default="all",
                     help="time only one form (for profiling runs)")
     ap.add_argument("--systems", type=int, default=4, help="systems per batch of the headline form")
     ap.add_argument("--gpus", type=int, default=1)
@@ -332,8 +341,9 @@ def main():
         sys.exit(2)
     warmup = max(1, args.warmup)
     nsys = args.systems
-    if args.form in ("string", "scan"):  # one extras line alone (for profiling runs)
-        print(json.dumps(run_string() if args.form == "string" else run_scan()), flush=True)
+    if args.form in ("string", "scan", "etdrk4"):  # one extras line alone (for profiling runs)
+        run = {"string": run_string, "scan": run_scan, "etdrk4": run_etdrk4}[args.form]
+        print(json.dumps(run()), flush=True)
         return
 
     batch = mixed = f64 = first32 = first64 = None
@@ -369,12 +379,7 @@ def main():
         print(f"[bench] string {json.dumps(extras['string_batched'])}", file=sys.stderr, flush=True)
         extras["ragged_scan"] = run_scan()
         print(f"[bench] scan {json.dumps(extras['ragged_scan'])}", file=sys.stderr, flush=True)
-        etd, _ = run_form("fp64", 10, 2, adjoint=False, algo="ETDRK4")
-        extras["etdrk4_fp64"] = {
-            "workload": "the same 128^3 Ns=200 system with the ETDRK4 updater (Etdrk4_PBC), FP64",
-            "contour_steps_per_s": etd["contour_steps_per_s"], "iters_per_s": etd["iters_per_s"],
-            "lockstep_step_us": etd["lockstep_step_us"], "achieved_GBs": etd["achieved_GBs"],
-            "bytes_per_launch": etd["bytes_per_launch"], "H": etd["H"]}
+        extras["etdrk4_fp64"] = run_etdrk4()
         print(f"[bench] etdrk4 {json.dumps(extras['etdrk4_fp64'])}", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu_baseline else cpu_baseline()
     # torch brings its own HIP runtime: the copy is timed in a child process

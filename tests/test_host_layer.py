@@ -382,3 +382,23 @@ def test_string_driver_reads_saved_string(tmp_path, monkeypatch):
     assert np.array_equal(r["wa"].reshape((m,) + shape), s.get("wA").reshape((m,) + shape))
     phia_out = r["phia"].reshape((m,) + shape)
     assert np.array_equal(phia_out[0], phia[0]) and np.array_equal(phia_out[-1], phia[-1])
+
+
+def test_string_driver_refuses_etdrk4_with_change_size(tmp_path):
+    """polystring_AB_amd refuses [Algorithm_MDE] algorithm = ETDRK4 together
+    with is_changeSize (per-image cells need per-image ETDRK4 coefficient
+    tables, which are not built).  The check runs before the first call into
+    the device library, so it needs no GPU."""
+    exe = os.path.join(PKG, "polystring_AB_amd")
+    if not os.path.exists(exe):
+        pytest.skip("polystring_AB_amd is not built (python -m polystring_amd.build)")
+    rng = np.random.default_rng(1)
+    for name in ("defect.mat", "vLam.mat"):
+        wA = 7.5 + 0.6 * rng.standard_normal((32, 32))
+        sio.savemat(str(tmp_path / name), {"wA": wA, "wB": 15.0 - wA, "phiA": np.full((32, 32), 0.5),
+                                           "phiB": np.full((32, 32), 0.5)})
+    (tmp_path / "paramString.ini").write_text(STRING_INI.replace("algorithm = RQM4", "algorithm = ETDRK4"))
+    out = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert out.returncode == 1, out.stdout + out.stderr
+    assert "is_changeSize" in out.stderr and "ETDRK4" in out.stderr
+    assert not (tmp_path / "result.mat").exists()

@@ -209,7 +209,7 @@ typedef struct {
     double chiN;
     double lamA, lamB;   /* lam[0], lam[1]: field mixing */
     double lamY;         /* lam[2]: yita mixing, or the compressibility penalty */
-    int algo;            /* PO_RQM4, PO_OS2 or PO_ETDRK4 (ETDRK4: no per-image sizes) */
+    int algo;            /* PO_RQM4, PO_OS2 or PO_ETDRK4 */
     int contour;         /* PO_QUAD_AUTO (Simpson / Quad4_Closed) or PO_QUAD_TRAP */
     int compressible;    /* is_compressible (src/Model_AB.cc:205-229) */
     int mirror;          /* confine_mold = NBC_by_PBC (src/Model_AB.cc:148-182) */

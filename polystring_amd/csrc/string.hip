@@ -262,6 +262,7 @@ template <typename T> struct StringAB : po_string {
     KTable<T> kA, kAh, kB, kBh;
     EtdTables<T> etdA, etdB;
     std::unique_ptr<DBuf<cx<T>>[]> etdState;  // ETDRK4: v, acc, t of each of the 2m lanes
+    std::vector<std::unique_ptr<EtdTables<T>>> etdimg;  // ETDRK4 after set_sizes: image i's A (2i), B (2i+1)
     // per-image cells (set_sizes): image i's tables kimg[4i .. 4i+3] = A, A half, B, B half
     std::vector<std::unique_ptr<KTable<T>>> kimg;
     Program<T> prog;
@@ -378,7 +379,12 @@ template <typename T> struct StringAB : po_string {
                 ln.Sf = spec.p + (size_t)(4 * i + 2 * l) * sh.Mh;
                 ln.Sh = spec.p + (size_t)(4 * i + 2 * l + 1) * sh.Mh;
                 if (desc.algo == PO_ETDRK4) {
-                    for (auto &sg : ln.segs) sg.etd = (sg.q == a || sg.q == ac || dsA == dsB) ? &etdA : &etdB;
+                    const EtdTables<T> *ea = &etdA, *eb = dsA == dsB ? &etdA : &etdB;
+                    if (!etdimg.empty()) {  // after set_sizes: image i's own cell
+                        ea = etdimg[2 * i].get();
+                        eb = dsA == dsB ? ea : etdimg[2 * i + 1].get();
+                    }
+                    for (auto &sg : ln.segs) sg.etd = (sg.q == a || sg.q == ac) ? ea : eb;
                     ln.ev = etdState[6 * i + 3 * l].p;
                     ln.eacc = etdState[6 * i + 3 * l + 1].p;
                     ln.et = etdState[6 * i + 3 * l + 2].p;
@@ -394,10 +400,11 @@ template <typename T> struct StringAB : po_string {
     // (1-D and 2-D strings) or b (3-D); the crystal system ties the other
     // lengths as UnitCell does.  The launch shape stays the tuned one.
     int set_sizes(const double *size) override {
-        if (desc.algo == PO_ETDRK4) return fail(PO_ERR_INVALID, "per-image cell sizes are not built for ETDRK4");
         hipStream_t st = ctx->stream;
         PO_HIP(hipStreamSynchronize(st));
-        std::vector<std::unique_ptr<KTable<T>>> k(4 * (size_t)m);
+        const bool etd = desc.algo == PO_ETDRK4;
+        std::vector<std::unique_ptr<KTable<T>>> k(etd ? 0 : 4 * (size_t)m);
+        std::vector<std::unique_ptr<EtdTables<T>>> e(etd ? 2 * (size_t)m : 0);
         const bool general = (desc.algo == PO_OS2) && !cell.ortho;
         int rc;
         for (int i = 0; i < m; ++i) {
@@ -407,6 +414,15 @@ template <typename T> struct StringAB : po_string {
             Cell ci{};
             if ((rc = make_cell(g, ci))) return rc;
             if (ci.ortho != cell.ortho) return fail(PO_ERR_INVALID, "image cells must keep the string's cell type");
+            if (etd) {  // Etdrk4_PBC re-initialised for the image's cell (src/Etdrk4_PBC.cc:43-90)
+                e[2 * i].reset(new EtdTables<T>());
+                if ((rc = build_etd_tables<T>(*e[2 * i], sh, ci, dsA, st))) return rc;
+                if (dsA != dsB) {
+                    e[2 * i + 1].reset(new EtdTables<T>());
+                    if ((rc = build_etd_tables<T>(*e[2 * i + 1], sh, ci, dsB, st))) return rc;
+                }
+                continue;
+            }
             for (int t = 0; t < 4; ++t) k[4 * i + t].reset(new KTable<T>());
             if ((rc = build_ktable<T>(*k[4 * i], sh, ci, general, dsA, st))) return rc;
             if ((rc = build_ktable<T>(*k[4 * i + 2], sh, ci, general, dsB, st))) return rc;
@@ -416,7 +432,9 @@ template <typename T> struct StringAB : po_string {
             }
         }
         PO_HIP(hipStreamSynchronize(st));
-        kimg.swap(k);  // the previous per-image tables (now in k) are freed after the rebuild
+        // the previous per-image tables (now in k or e) are freed after the rebuild
+        if (etd) etdimg.swap(e);
+        else kimg.swap(k);
         rc = build_program();
         PO_HIP(hipStreamSynchronize(st));
         return rc;

@@ -157,18 +157,13 @@ int main(int argc, char *argv[]) {
     d.algo = cfg.algo_mde_type() == AlgorithmMDEType::OS2    ? PO_OS2
              : cfg.algo_mde_type() == AlgorithmMDEType::RQM4 ? PO_RQM4
                                                               : PO_ETDRK4;
-    if (d.algo == PO_ETDRK4 && is_changeSize) {
-        std::cerr << "polystring: per-image cell sizes (is_changeSize) are not built for ETDRK4" << std::endl;
-        return 1;
-    }
     d.contour = cfg.algo_contour_integration_type() == AlgorithmContourType::TRAPEZOIDAL ? PO_QUAD_TRAP : PO_QUAD_AUTO;
     d.compressible = cfg.is_compressible() ? 1 : 0;
     d.mirror = lower(cfg.get_string("Grid", "confine_mold")) == "nbc_by_pbc" ? 1 : 0;
     d.m = m;
     po_string *str = nullptr;
     check(po_string_create(po_host_context(&cfg), &d, wa.data(), wb.data(), &str), "po_string_create");
-    // ETDRK4 keeps the configured cell for every image (no per-image tables)
-    if (d.algo != PO_ETDRK4) check(po_string_set_sizes(str, stringSize.data()), "po_string_set_sizes");
+    check(po_string_set_sizes(str, stringSize.data()), "po_string_set_sizes");
 
     // scft::run of each image: [Algorithm_SCFT] min_iter, max_iter, thresh_residual; [IO] record_interval
     const int max_iter = (int)cfg.max_iter(), min_iter = (int)cfg.min_iter();

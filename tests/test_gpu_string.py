@@ -116,22 +116,24 @@ SIZE_CASES = [
 ]
 
 
+@pytest.mark.parametrize("algo", ["RQM4", "ETDRK4"])
 @pytest.mark.parametrize("shape,cellf,dim,Ms,compressible,mirror,lam", SIZE_CASES)
-def test_string_per_image_sizes_match_oracle(ctx64, shape, cellf, dim, Ms, compressible, mirror, lam):
+def test_string_per_image_sizes_match_oracle(ctx64, shape, cellf, dim, Ms, compressible, mirror, lam, algo):
     """is_changeSize (scft/polystring_AB.cc:85-91, src/Model_AB.cc:52-83):
     image s takes size_S + s/(m-1) (size_E - size_S) as its cell length a
-    (1-D, 2-D) or b (3-D).  The oracle runs each image with that cell."""
+    (1-D, 2-D) or b (3-D).  The oracle runs each image with that cell.  With
+    ETDRK4 each image's Etdrk4_PBC coefficients follow its own cell."""
     m, chiN = 5, 15.0
     size_S, size_E = 3.7, 4.5
     sizes = size_S + np.arange(m) / (m - 1) * (size_E - size_S)
-    print(f"\n[string sizes] {shape} sizes {sizes.tolist()}", flush=True)
+    print(f"\n[string sizes] {algo} {shape} sizes {sizes.tolist()}", flush=True)
     wa, wb = initial_string(shape, m, chiN, 0.5)
     relax = (3, 1, 1, 1e-10)
     models = [O.EMModel(wa[s], wb[s], dim=dim, cell=cellf(sizes[s]), Ms=Ms, chiN=chiN, lam=lam, mirror=mirror,
-                        compressible=compressible) for s in range(m)]
+                        compressible=compressible, mde=algo) for s in range(m)]
     ends = (wa[0].copy(), wb[0].copy(), wa[-1].copy(), wb[-1].copy())
     g = capi.String(ctx64, wa, wb, cell=cellf(4.0), Ms=Ms, chiN=chiN, lam=lam, compressible=compressible,
-                    mirror=mirror, dim=dim)
+                    mirror=mirror, dim=dim, algo=algo)
     g.set_sizes(sizes)
     owa, owb = wa, wb
     for it in range(2):
@@ -144,7 +146,7 @@ def test_string_per_image_sizes_match_oracle(ctx64, shape, cellf, dim, Ms, compr
         assert relerr(g.get("phiA"), opa) <= 1e-10
     # the sizes matter: the string's images now differ from a string at the base cell
     base = capi.String(ctx64, wa, wb, cell=cellf(4.0), Ms=Ms, chiN=chiN, lam=lam, compressible=compressible,
-                       mirror=mirror, dim=dim)
+                       mirror=mirror, dim=dim, algo=algo)
     Hb, _ = base.iterate(*relax)
     assert relerr(Hb, Ho) > 1e-6
     with pytest.raises(capi.POError):
@@ -192,5 +194,3 @@ def test_string_etdrk4_matches_oracle(ctx64, shape, cell, dim, Ms, compressible,
         assert np.abs(S - So).max() <= 1e-10
         assert relerr(g.get("wA"), owa) <= 1e-10
         assert relerr(g.get("phiA"), opa) <= 1e-10
-    with pytest.raises(capi.POError):
-        g.set_sizes(np.full(m, 4.0))

@@ -116,13 +116,14 @@ SIZE_CASES = [
 ]
 
 
-@pytest.mark.parametrize("algo", ["RQM4", "ETDRK4"])
+@pytest.mark.parametrize("algo", ["RQM4", "ETDRK4", "OS2"])
 @pytest.mark.parametrize("shape,cellf,dim,Ms,compressible,mirror,lam", SIZE_CASES)
 def test_string_per_image_sizes_match_oracle(ctx64, shape, cellf, dim, Ms, compressible, mirror, lam, algo):
     """is_changeSize (scft/polystring_AB.cc:85-91, src/Model_AB.cc:52-83):
     image s takes size_S + s/(m-1) (size_E - size_S) as its cell length a
     (1-D, 2-D) or b (3-D).  The oracle runs each image with that cell.  With
-    ETDRK4 each image's Etdrk4_PBC coefficients follow its own cell."""
+    ETDRK4 each image's Etdrk4_PBC coefficients follow its own cell, with OS2
+    (PseudoSpectral) its exp(-k^2 ds) table."""
     m, chiN = 5, 15.0
     size_S, size_E = 3.7, 4.5
     sizes = size_S + np.arange(m) / (m - 1) * (size_E - size_S)
